@@ -1,0 +1,115 @@
+/*
+ * swit_fd.h -- C ABI of the MI355X (gfx950) 2-D acoustic forward/adjoint solver.
+ *
+ * The reference has no in-process FFI: its boundary is the process call
+ *     mpirun -np N fd2dmpi par=<parfile>          (toolbox/solver.py:54,79)
+ * whose jobs are forward_modeling (fd2dmpi/solver.f90:36) and adjoint_modeling
+ * (fd2dmpi/solver.f90:120).  The entry points below replace that process boundary
+ * one-for-one; the Python drop-in (swit_amd/solver.py) binds them with ctypes.
+ *
+ * Conventions
+ *   - Every array pointer is DEVICE memory unless stated otherwise; the caller owns
+ *     inputs and outputs, the plan owns its scratch (wavefields, strips).
+ *   - 2-D model/gradient arrays are [nx][nz] with iz fastest: the byte layout of the
+ *     reference's vel.bin / rho.bin / src0_kernel_vp.bin (fd2dmpi/io.f90:231-264,
+ *     :590-606) and of numpy (nx, nz).
+ *   - Return 0 on success; a nonzero SWIT_E* code otherwise, with a message in
+ *     swit_last_error() (thread-local).
+ *   - One plan per stream; calls on one plan are serialised by the caller.
+ */
+#ifndef SWIT_FD_H
+#define SWIT_FD_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes. */
+#define SWIT_OK 0
+#define SWIT_EINVAL 1       /* bad grid / geometry / argument                       */
+#define SWIT_ENOMEM 2       /* device allocation failed                             */
+#define SWIT_EHIP 3         /* a HIP runtime call failed                            */
+#define SWIT_ESTATE 4       /* adjoint without a matching strip-saving forward      */
+
+/* Plan flags. */
+#define SWIT_PATH_PLAIN 0x0     /* per-shot, per-step kernel launches               */
+#define SWIT_PATH_BATCHED 0x1   /* all shots per launch, fused steps, graph-captured */
+#define SWIT_STRIPS_F16 0x10    /* boundary strips stored as FP16 with per-step scale */
+#define SWIT_STRIPS_BF16 0x20   /* boundary strips stored as BF16 with per-step scale */
+
+/* Grid and time axis: the reference's param type (fd2dmpi/datatype.f90:17-22) as
+ * read from the parfile (io.f90:72-96).  dx and dt are the float32 values the
+ * reference reads (the %f text of toolbox/solver.py:145-149). */
+typedef struct swit_grid {
+    int32_t nx, nz;         /* interior grid (NX, NZ)                                */
+    int32_t npml;           /* absorbing-layer width (NPML, default 40)              */
+    int32_t nt;             /* time samples (NT_WORK)                                */
+    int32_t free_surface;   /* FREESURFACE; the reference forces the top row anyway  */
+    float dx, dt;           /* DX, DT_WORK                                           */
+} swit_grid;
+
+/* Acquisition: the reference's acquisition type (datatype.f90:24-30) as read from
+ * coord.txt (io.f90:126-228).  HOST pointers, copied at plan creation. */
+typedef struct swit_shots {
+    int32_t ns;             /* number of shots                                       */
+    int32_t ngmax;          /* max receivers per shot                                */
+    const int32_t *ng;      /* [ns] receivers per shot                               */
+    const float *src_xz;    /* [ns][2] source x, z in metres                         */
+    const float *rec_xz;    /* [ns][ngmax][2] receiver x, z in metres                */
+} swit_shots;
+
+typedef struct swit_plan swit_plan;
+
+/* Allocates the plan: wavefields, strips, accumulators for all shots.
+ * Replaces the setup half of forward_modeling/adjoint_modeling
+ * (fd2dmpi/solver.f90:47-76 and :131-178): grid extents (pml.f90:24-38), coordinate
+ * to cell mapping (fdcore.f90:70-72,124-126,216-218,350-352), free-surface row
+ * (io.f90:777-800). */
+int swit_plan_create(const swit_grid *grid, const swit_shots *shots, int flags,
+                     swit_plan **out);
+
+/* Loads vp/rho ([nx][nz] float32, device) into the plan: PML padding
+ * (io.f90:231-300), vmin (math.f90:342-361, solver.f90:73), damping profile
+ * (pml.f90:74-101), and the per-cell coefficients den*c^2*dt/dx and dt/(den*dx)
+ * (fdcore.f90:79,99,108).  Invalidates saved strips. */
+int swit_plan_set_model(swit_plan *plan, const float *vp, const float *rho,
+                        hipStream_t stream);
+
+/* Forward job for every shot of the plan: replaces forward_modeling
+ * (solver.f90:36-116) / staggered42_modeling_is (fdcore.f90:30-198).
+ *   src:  [ns][nt] integrated source traces (the reference's src%d.bin).
+ *   seis: [ns][ngmax][nt] pressure seismograms (the SU trace data of src%d_sg.su),
+ *         sample 0 is zero as in the reference.
+ *   save_strips: 1 keeps boundary strips and the final state on the device for a
+ *         following swit_adjoint (the reference's STORE_SNAP=1 files,
+ *         fdcore.f90:130-176); 0 invalidates them (the reference's cleandata). */
+int swit_forward(swit_plan *plan, const float *src, float *seis, int save_strips,
+                 hipStream_t stream);
+
+/* Adjoint job for every shot: replaces adjoint_modeling (solver.f90:120-289):
+ * reverse-time reconstruction (fdcore.f90:202-289), adjoint propagation
+ * (fdcore.f90:293-375), zero-lag imaging (solver.f90:237-239), cross-shot sum
+ * (solver.f90:263-265, here FP64 on the device) and g = -2 g / c (solver.f90:270).
+ *   src:    [ns][nt] the same integrated source traces as the forward.
+ *   adjsrc: [ns][ngmax][nt] adjoint sources (the reference's adjoint src%d.bin).
+ *   grad, illum_forw, illum_back: [nx][nz] float64 outputs (src0_kernel_vp.bin,
+ *         src0_illum_forw.bin, src0_illum_back.bin). */
+int swit_adjoint(swit_plan *plan, const float *src, const float *adjsrc, double *grad,
+                 double *illum_forw, double *illum_back, hipStream_t stream);
+
+/* Releases every device buffer of the plan. */
+int swit_plan_destroy(swit_plan *plan);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *swit_last_error(void);
+
+/* Bytes of device scratch the plan holds (wavefields, strips, accumulators). */
+int64_t swit_plan_scratch_bytes(const swit_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SWIT_FD_H */

@@ -1,0 +1,106 @@
+"""ctypes binding of the C ABI in include/swit_fd.h (libswit_fd.so, gfx950).
+
+The library is built in-tree by ``swit_amd.build`` (hipcc, --offload-arch=gfx950).
+There is no fallback: if the library is missing, loading fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libswit_fd.so")
+
+# Error codes and flags, include/swit_fd.h
+SWIT_OK = 0
+SWIT_EINVAL = 1
+SWIT_ENOMEM = 2
+SWIT_EHIP = 3
+SWIT_ESTATE = 4
+SWIT_PATH_PLAIN = 0x0
+SWIT_PATH_BATCHED = 0x1
+SWIT_STRIPS_F16 = 0x10
+SWIT_STRIPS_BF16 = 0x20
+
+# Every entry point include/swit_fd.h declares.
+EXPORTS = (
+    "swit_plan_create",
+    "swit_plan_set_model",
+    "swit_forward",
+    "swit_adjoint",
+    "swit_plan_destroy",
+    "swit_last_error",
+    "swit_plan_scratch_bytes",
+)
+
+
+class SwitGrid(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int32),
+        ("nz", ctypes.c_int32),
+        ("npml", ctypes.c_int32),
+        ("nt", ctypes.c_int32),
+        ("free_surface", ctypes.c_int32),
+        ("dx", ctypes.c_float),
+        ("dt", ctypes.c_float),
+    ]
+
+
+class SwitShots(ctypes.Structure):
+    _fields_ = [
+        ("ns", ctypes.c_int32),
+        ("ngmax", ctypes.c_int32),
+        ("ng", ctypes.POINTER(ctypes.c_int32)),
+        ("src_xz", ctypes.POINTER(ctypes.c_float)),
+        ("rec_xz", ctypes.POINTER(ctypes.c_float)),
+    ]
+
+
+class NativeLibraryMissing(ImportError):
+    pass
+
+
+class SwitError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"swit error {code}: {message}")
+        self.code = code
+
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load libswit_fd.so and declare its signatures (no GPU call is made)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise NativeLibraryMissing(
+            f"{p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = ctypes.CDLL(p)
+    vp = ctypes.c_void_p
+    L.swit_plan_create.argtypes = [ctypes.POINTER(SwitGrid), ctypes.POINTER(SwitShots), ctypes.c_int,
+                                   ctypes.POINTER(vp)]
+    L.swit_plan_create.restype = ctypes.c_int
+    L.swit_plan_set_model.argtypes = [vp, vp, vp, vp]
+    L.swit_plan_set_model.restype = ctypes.c_int
+    L.swit_forward.argtypes = [vp, vp, vp, ctypes.c_int, vp]
+    L.swit_forward.restype = ctypes.c_int
+    L.swit_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.swit_adjoint.restype = ctypes.c_int
+    L.swit_plan_destroy.argtypes = [vp]
+    L.swit_plan_destroy.restype = ctypes.c_int
+    L.swit_last_error.argtypes = []
+    L.swit_last_error.restype = ctypes.c_char_p
+    L.swit_plan_scratch_bytes.argtypes = [vp]
+    L.swit_plan_scratch_bytes.restype = ctypes.c_int64
+    if path is None:
+        _lib = L
+    return L
+
+
+def check(rc: int):
+    if rc != SWIT_OK:
+        msg = load().swit_last_error().decode(errors="replace")
+        raise SwitError(rc, msg)

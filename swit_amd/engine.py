@@ -1,0 +1,136 @@
+"""Device plan: owns one libswit_fd plan and moves torch device buffers through it.
+
+PyTorch provides device memory and the stream; all solver arithmetic runs in the
+HIP kernels of libswit_fd.so.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+
+
+def _require_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("swit_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def pack_receivers(rec_xz, ns):
+    """(ns, ng, 2) array or list of (ng_i, 2) arrays -> (ng[ns], rec[ns, ngmax, 2])."""
+    if isinstance(rec_xz, (list, tuple)):
+        recs = [np.asarray(r, dtype=np.float32).reshape(-1, 2) for r in rec_xz]
+    else:
+        a = np.asarray(rec_xz, dtype=np.float32)
+        if a.ndim == 2:
+            a = np.broadcast_to(a, (ns,) + a.shape)
+        recs = [a[i].reshape(-1, 2) for i in range(ns)]
+    if len(recs) != ns:
+        raise ValueError(f"receiver list has {len(recs)} shots, expected {ns}")
+    ng = np.array([r.shape[0] for r in recs], dtype=np.int32)
+    ngmax = int(ng.max()) if ns else 0
+    out = np.zeros((ns, max(ngmax, 1), 2), dtype=np.float32)
+    for i, r in enumerate(recs):
+        out[i, : r.shape[0]] = r
+    return ng, out
+
+
+class Plan:
+    """One solver plan: grid, acquisition and device scratch for every shot.
+
+    path: 'plain' (per-shot, per-step launches) or 'batched' (all shots per launch).
+    strips: 'f32' | 'f16' | 'bf16' storage of the boundary strips.
+    """
+
+    def __init__(self, nx, nz, dx, dt, nt, npml, src_xz, rec_xz, free_surface=1,
+                 path="batched", strips="f32"):
+        torch = _require_gpu()
+        self._torch = torch
+        self._lib = _native.load()
+        src_xz = np.ascontiguousarray(np.asarray(src_xz, dtype=np.float32).reshape(-1, 2))
+        self.ns = src_xz.shape[0]
+        ng, rec = pack_receivers(rec_xz, self.ns)
+        self.ng = ng
+        self.ngmax = rec.shape[1]
+        self.nx, self.nz, self.nt, self.npml = int(nx), int(nz), int(nt), int(npml)
+        self.dx, self.dt = float(np.float32(dx)), float(np.float32(dt))
+        flags = {"plain": _native.SWIT_PATH_PLAIN, "batched": _native.SWIT_PATH_BATCHED}[path]
+        flags |= {"f32": 0, "f16": _native.SWIT_STRIPS_F16, "bf16": _native.SWIT_STRIPS_BF16}[strips]
+        self.path, self.strips = path, strips
+        g = _native.SwitGrid(self.nx, self.nz, self.npml, self.nt, int(free_surface),
+                             self.dx, self.dt)
+        self._keep = (np.ascontiguousarray(ng), src_xz, np.ascontiguousarray(rec))
+        sh = _native.SwitShots(
+            self.ns, self.ngmax,
+            self._keep[0].ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            self._keep[1].ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+            self._keep[2].ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        h = ctypes.c_void_p()
+        torch.cuda.init()
+        _native.check(self._lib.swit_plan_create(ctypes.byref(g), ctypes.byref(sh), flags,
+                                                 ctypes.byref(h)))
+        self._h = h
+        self.device = torch.device("cuda", torch.cuda.current_device())
+
+    # -- helpers ----------------------------------------------------------------
+    def _stream(self):
+        return ctypes.c_void_p(self._torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _dev(self, x, dtype, shape):
+        torch = self._torch
+        t = torch.as_tensor(x, dtype=dtype, device=self.device).contiguous()
+        if tuple(t.shape) != tuple(shape):
+            raise ValueError(f"expected shape {tuple(shape)}, got {tuple(t.shape)}")
+        return t
+
+    @property
+    def scratch_bytes(self):
+        return int(self._lib.swit_plan_scratch_bytes(self._h))
+
+    # -- jobs ---------------------------------------------------------------------
+    def set_model(self, vp, rho):
+        torch = self._torch
+        self._vp = self._dev(vp, torch.float32, (self.nx, self.nz))
+        self._rho = self._dev(rho, torch.float32, (self.nx, self.nz))
+        _native.check(self._lib.swit_plan_set_model(self._h, ctypes.c_void_p(self._vp.data_ptr()),
+                                                    ctypes.c_void_p(self._rho.data_ptr()),
+                                                    self._stream()))
+
+    def forward(self, src, save_strips=True, out=None):
+        """src: (ns, nt) integrated source traces -> seismograms (ns, ngmax, nt)."""
+        torch = self._torch
+        src = self._dev(src, torch.float32, (self.ns, self.nt))
+        if out is None:
+            out = torch.empty((self.ns, self.ngmax, self.nt), dtype=torch.float32, device=self.device)
+        _native.check(self._lib.swit_forward(self._h, ctypes.c_void_p(src.data_ptr()),
+                                             ctypes.c_void_p(out.data_ptr()), int(bool(save_strips)),
+                                             self._stream()))
+        self._src = src
+        return out
+
+    def adjoint(self, src, adjsrc):
+        """-> (grad, illum_forw, illum_back), each (nx, nz) float64 on the device."""
+        torch = self._torch
+        src = self._dev(src, torch.float32, (self.ns, self.nt))
+        adjsrc = self._dev(adjsrc, torch.float32, (self.ns, self.ngmax, self.nt))
+        outs = [torch.empty((self.nx, self.nz), dtype=torch.float64, device=self.device) for _ in range(3)]
+        _native.check(self._lib.swit_adjoint(self._h, ctypes.c_void_p(src.data_ptr()),
+                                             ctypes.c_void_p(adjsrc.data_ptr()),
+                                             *[ctypes.c_void_p(o.data_ptr()) for o in outs],
+                                             self._stream()))
+        return tuple(outs)
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.swit_plan_destroy(h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
