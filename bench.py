@@ -1,0 +1,204 @@
+"""Benchmark: shot-gradients/s of the 2-D acoustic FWI gradient on one MI355X.
+
+Workload (default): 481 x 141 grid, dx 25 m, 40-cell PML, nt 4000, dt 1.9 ms, 49 shots,
+481 surface receivers, seeded dipping-layer + fault model (swit_amd.workloads.config2).
+One step = one full L2 gradient over every shot: model upload, forward with strip
+saving and receiver sampling, adjoint source (syn - obs), reverse-time reconstruction,
+adjoint propagation, imaging and the cross-shot sum.  Inputs (models, sources,
+observed data) are resident in HBM before the timed region.
+
+Prints one JSON line with the metric, a roofline object for the dominant kernel (the
+backward step kernel, timed with HIP events on its own stream) and a CPU baseline
+(the FP32 oracle on the host cores, bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+if REPO not in sys.path:
+    sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(nx, nz, npml, ng):
+    """Per shot and time step: (forward, backward) algorithmic bytes.
+    forward  = 36 N_pad (p,u,w read+write, coefficients) + 4 S (strip write) + 4 ng
+    backward = 36 N_pad (adjoint) + 32 N_int (reconstruction) + 24 N_int (imaging)
+               + 4 S (strip read) + 4 ng (adjoint source read)."""
+    n_pad = (nx + 2 * npml) * (nz + 2 * npml)
+    n_int = nx * nz
+    S = 9 * nx + 6 * nz
+    fwd = 36 * n_pad + 4 * S + 4 * ng
+    bwd = 36 * n_pad + 32 * n_int + 24 * n_int + 4 * S + 4 * ng
+    return fwd, bwd
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (profiles/traffic_<kernel>.json: FETCH_SIZE and WRITE_SIZE in KiB per dispatch,
+    scaled by the copy-kernel calibration factor), or None."""
+    p = os.path.join(REPO, "profiles", f"traffic_{kernel}.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        t = json.load(f)
+    return t.get("bytes_per_launch")
+
+
+def cpu_baseline(w, src, obs_syn_adj, nshots):
+    """FP32 oracle (C restatement of the reference scheme) on host cores: nshots
+    shot-gradients of the same workload, one thread per shot."""
+    from oracle import oracle as O
+    threads = max(1, min(nshots, os.cpu_count() or 1, 16))
+    rho = w.rho(w.vp_init)
+    t0 = time.perf_counter()
+    m = O.Model(w.vp_init, rho, w.dx, w.npml, "f32")
+    sel = slice(0, nshots)
+    f = O.forward(m, w.nt, w.dt, w.src_xz[sel], w.rec_xz[sel], src[sel], threads=threads)
+    adj = obs_syn_adj(f["seis"])
+    O.adjoint(m, w.nt, w.dt, w.src_xz[sel], w.rec_xz[sel], src[sel], adj, f, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": nshots / dt, "unit": "shot-gradients/s", "cores": threads, "kind": "port",
+            "sample": f"{nshots} shots of the same workload (full nt), FP32 oracle, one thread per shot, "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nt", type=int, default=4000)
+    ap.add_argument("--path", default="batched", choices=["batched", "plain"])
+    ap.add_argument("--strips", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-shots", type=int, default=0, help="CPU baseline sample size (0: auto)")
+    ap.add_argument("--plain-shots", type=int, default=2, help="plain-path comparison sample (0: skip)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
+              "(shots shard across GPUs with one all-reduce per gradient; not built here)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from swit_amd import workloads
+    from swit_amd.engine import Plan
+    from scipy.integrate import cumulative_trapezoid
+
+    w = workloads.config2(nt=args.nt)
+    wav = workloads.ricker(w.nt, w.dt, w.f0)
+    src_np = np.broadcast_to(cumulative_trapezoid(wav, initial=0).astype(np.float32), (w.ns, w.nt)).copy()
+    dev = torch.device("cuda", 0)
+    src = torch.from_numpy(src_np).to(dev)
+    vp_true = torch.from_numpy(w.vp_true).to(dev)
+    vp_init = torch.from_numpy(w.vp_init).to(dev)
+    rho_true = torch.from_numpy(w.rho(w.vp_true)).to(dev)
+    rho_init = torch.from_numpy(w.rho(w.vp_init)).to(dev)
+
+    plan = Plan(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, w.src_xz, w.rec_xz, path=args.path, strips=args.strips)
+    plan.set_model(vp_true, rho_true)
+    obs = plan.forward(src, save_strips=False).clone()
+    torch.cuda.synchronize()
+
+    def step():
+        plan.set_model(vp_init, rho_init)
+        syn = plan.forward(src, save_strips=True)
+        adj = syn - obs
+        return plan.adjoint(src, adj)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g = step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1000.0 * elapsed / args.steps
+    value = w.ns * args.steps / elapsed
+
+    # kernel timing pass: HIP events around every step kernel, on the launch stream
+    roofline = None
+    if args.path == "batched":
+        plan.set_option("profile", 1)
+        step()
+        torch.cuda.synchronize()
+        f_ms, f_n = plan.kernel_time("forward")
+        b_ms, b_n = plan.kernel_time("backward")
+        plan.set_option("profile", 0)
+        fwd_b, bwd_b = algorithmic_bytes(w.nx, w.nz, w.npml, w.ng)
+        kern = {"forward": (f_ms / max(f_n, 1), fwd_b * w.ns, "k_fwd_fused"),
+                "backward": (b_ms / max(b_n, 1), bwd_b * w.ns, "k_bwd_fused")}
+        dom = max(kern, key=lambda k: kern[k][0] * (f_n if k == "forward" else b_n))
+        avg_ms, bytes_per_launch, kname = kern[dom]
+        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(kname),
+                    "kernel": kname, "avg_launch_us": round(avg_ms * 1e3, 2),
+                    "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                    "other_kernel": {
+                        "name": kern["forward" if dom == "backward" else "backward"][2],
+                        "avg_launch_us": round(kern["forward" if dom == "backward" else "backward"][0] * 1e3, 2),
+                        "achieved_gbs": round(kern["forward" if dom == "backward" else "backward"][1]
+                                              / (kern["forward" if dom == "backward" else "backward"][0] * 1e-3)
+                                              / 1e9, 1)}}
+
+    plain = None
+    if args.plain_shots > 0 and args.path == "batched":
+        k = min(args.plain_shots, w.ns)
+        pp = Plan(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, w.src_xz[:k], w.rec_xz[:k], path="plain")
+        pp.set_model(vp_true, rho_true)
+        obs_k = pp.forward(src[:k], save_strips=False).clone()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        pp.set_model(vp_init, rho_init)
+        syn_k = pp.forward(src[:k], save_strips=True)
+        pp.adjoint(src[:k], syn_k - obs_k)
+        torch.cuda.synchronize()
+        pv = k / (time.perf_counter() - t1)
+        pp.close()
+        plain = {"value": round(pv, 3), "unit": "shot-gradients/s", "shots": k,
+                 "batched_over_plain": round(value / pv, 2)}
+
+    cpu = None
+    if not args.no_cpu:
+        obs_np = obs.cpu().numpy()
+        n = args.cpu_shots or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(w, src_np, lambda syn: (syn - obs_np[: syn.shape[0]]).astype(np.float32), n)
+
+    line = {
+        "metric": "shot-gradients/s at 481x141x4000 steps (batched); GB/s vs HBM peak",
+        "value": round(value, 3),
+        "unit": "shot-gradients/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 12-layer dipping model with two faults, Ricker 5 Hz)",
+        "config": {"workload": f"{w.nx}x{w.nz} grid, npml {w.npml}, nt {w.nt}, dt {w.dt}, {w.ns} shots, "
+                               f"{w.ng} receivers, L2 gradient", "path": args.path, "strips": args.strips,
+                   "shots_per_step": w.ns},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain_baseline": plain,
+    }
+    print(json.dumps(line), flush=True)
+    plan.close()
+
+
+if __name__ == "__main__":
+    main()

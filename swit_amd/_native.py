@@ -31,7 +31,14 @@ EXPORTS = (
     "swit_plan_destroy",
     "swit_last_error",
     "swit_plan_scratch_bytes",
+    "swit_plan_set_option",
+    "swit_plan_kernel_time",
 )
+
+SWIT_OPT_GRAPH = 1
+SWIT_OPT_PROFILE = 2
+SWIT_OPT_BWD_GROUP = 3
+SWIT_OPT_FWD_GROUP = 4
 
 
 class SwitGrid(ctypes.Structure):
@@ -95,6 +102,11 @@ def load(path: str | None = None):
     L.swit_last_error.restype = ctypes.c_char_p
     L.swit_plan_scratch_bytes.argtypes = [vp]
     L.swit_plan_scratch_bytes.restype = ctypes.c_int64
+    L.swit_plan_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.swit_plan_set_option.restype = ctypes.c_int
+    L.swit_plan_kernel_time.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_int64)]
+    L.swit_plan_kernel_time.restype = ctypes.c_int
     if path is None:
         _lib = L
     return L

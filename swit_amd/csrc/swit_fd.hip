@@ -7,6 +7,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
+#include <cstdlib>
+#include <utility>
 #include <vector>
 
 #include "swit_fd.h"
@@ -48,6 +51,8 @@ struct swit_plan {
     int ns = 0;
     int flags = 0;
     std::vector<ShotGeom> geo_h;
+    std::vector<int> rec_cells_fwd, rec_cells_inj;   // host copies of rec_fwd / rec_inj
+    std::vector<uint32_t> strip_map_h;
     std::vector<void *> allocs;
     int64_t bytes = 0;
     // geometry
@@ -181,6 +186,9 @@ static int plan_build(swit_plan *pl, const swit_grid *g, const swit_shots *sh, i
         put(1, fs, j); put(1, fs + 1, j); put(2, fs, j);
     }
     if ((int)sm.size() != d.S) return set_err(SWIT_EINVAL, "internal: strip map size");
+    pl->rec_cells_fwd = rf;
+    pl->rec_cells_inj = ri;
+    pl->strip_map_h = sm;
 
     const size_t np = (size_t)d.nxp * d.ld, ni = (size_t)d.nx * d.nz;
     TRY(pl->alloc(&pl->geo, ns));
@@ -388,77 +396,365 @@ extern "C" int swit_adjoint(swit_plan *pl, const float *src, const float *adjsrc
 
 // ---- batched path (host side) --------------------------------------------------------
 
-static int batched_init(swit_plan *pl) {
-    pl->bat.ready = 1;
+static TileMap make_tilemap(const Dims &d, int ztop) {
+    TileMap tm;
+    tm.ztop = ztop;
+    tm.nzt = cdiv(d.nzp - ztop + 1, kTZ);
+    tm.nxt = cdiv(d.nxp, kTX);
+    tm.ntiles = tm.nzt * tm.nxt;
+    return tm;
+}
+
+static int tile_of(const TileMap &tm, int iz, int ix) {
+    int zt = (iz - tm.ztop) / kTZ, xt = (ix - 1) / kTX;
+    return xt * tm.nzt + zt;
+}
+
+static int fill(float *p, size_t n, hipStream_t st) {
+    int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, st, p, n, 0.0f);
+    LAUNCH_CHECK();
     return SWIT_OK;
 }
 
-static void batched_release(swit_plan *pl) { pl->bat.ready = 0; }
+static int copyk(const float *a, float *b, size_t n, hipStream_t st) {
+    int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, st, a, b, n);
+    LAUNCH_CHECK();
+    return SWIT_OK;
+}
 
-static int batched_set_model(swit_plan *, hipStream_t) { return SWIT_OK; }
-
-static int batched_forward(swit_plan *pl, const float *src, int save, hipStream_t st) {
+static int batched_init(swit_plan *pl) {
+    BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
-    const size_t np = (size_t)d.nxp * d.ld;
+    const int ns = pl->ns, ngmax = d.ngmax, fs = d.fs;
+    bs.fwd = make_tilemap(d, fs - 1);
+    // Backward tiles start at row fs-3 so rows fs-2..fs+1 share one tile: the adjoint
+    // p stencil at rows fs-2..fs+1 reads the mirrored w(fs-1), and the p mirror reads
+    // row fs+1 after injection (fdcore.f90:329-333, :370-373).
+    bs.bwd = make_tilemap(d, (fs - 3) - kTZ * cdiv(fs - 4, kTZ));
+    // receivers: recording cells (forward tiling) and injection cells (backward tiling)
+    std::vector<int> rs((size_t)ns * (bs.fwd.ntiles + 1)), is((size_t)ns * (bs.bwd.ntiles + 1));
+    std::vector<RecRef> rl((size_t)ns * ngmax), il((size_t)ns * ngmax);
+    std::vector<InjGroup> ig((size_t)ns * ngmax, InjGroup{0, 0});
+    for (int s = 0; s < ns; ++s) {
+        const ShotGeom &g = pl->geo_h[s];
+        (void)g;
+        std::vector<std::pair<int, RecRef>> a, b;
+        for (int r = 0; r < pl->geo_h[s].ng; ++r) {
+            int cf = pl->rec_cells_fwd[(size_t)s * ngmax + r], ci = pl->rec_cells_inj[(size_t)s * ngmax + r];
+            RecRef rf{r, cf % d.ld + 1, cf / d.ld + 1, 0}, ri{r, ci % d.ld + 1, ci / d.ld + 1, 0};
+            if (rf.iz < fs - 1) return set_err(SWIT_EINVAL, "shot %d receiver %d above the free surface", s, r);
+            a.push_back({tile_of(bs.fwd, rf.iz, rf.ix), rf});
+            b.push_back({tile_of(bs.bwd, ri.iz, ri.ix), ri});
+        }
+        auto by_tile = [](const std::pair<int, RecRef> &x, const std::pair<int, RecRef> &y) { return x.first < y.first; };
+        std::stable_sort(a.begin(), a.end(), by_tile);
+        std::stable_sort(b.begin(), b.end(), by_tile);
+        int *sa = &rs[(size_t)s * (bs.fwd.ntiles + 1)], *sb = &is[(size_t)s * (bs.bwd.ntiles + 1)];
+        for (int t = 0, k = 0; t <= bs.fwd.ntiles; ++t) {
+            while (k < (int)a.size() && a[k].first < t) ++k;
+            sa[t] = k;
+        }
+        for (int t = 0, k = 0; t <= bs.bwd.ntiles; ++t) {
+            while (k < (int)b.size() && b[k].first < t) ++k;
+            sb[t] = k;
+        }
+        for (size_t k = 0; k < a.size(); ++k) rl[(size_t)s * ngmax + k] = a[k].second;
+        for (size_t k = 0; k < b.size(); ++k) il[(size_t)s * ngmax + k] = b[k].second;
+        // injection entries of a tile sorted by cell (stable: receiver order kept),
+        // then grouped so one thread adds every receiver of a cell in order
+        std::vector<InjGroup> grs;
+        std::vector<int> ent(sb, sb + bs.bwd.ntiles + 1);   // entry ranges per tile
+        int *gs = sb;                                        // becomes group ranges per tile
+        for (int t = 0; t < bs.bwd.ntiles; ++t) {
+            int e0 = ent[t], e1 = ent[t + 1];
+            RecRef *base = &il[(size_t)s * ngmax];
+            std::stable_sort(base + e0, base + e1, [](const RecRef &x, const RecRef &y) {
+                return x.ix != y.ix ? x.ix < y.ix : x.iz < y.iz;
+            });
+            gs[t] = (int)grs.size();
+            for (int e = e0; e < e1;) {
+                int f = e + 1;
+                while (f < e1 && base[f].ix == base[e].ix && base[f].iz == base[e].iz) ++f;
+                grs.push_back(InjGroup{e, f});
+                e = f;
+            }
+        }
+        gs[bs.bwd.ntiles] = (int)grs.size();
+        for (size_t k = 0; k < grs.size(); ++k) ig[(size_t)s * ngmax + k] = grs[k];
+    }
+    // strip slots by forward tile
+    std::vector<std::pair<int, StripRef>> sv;
+    for (int k = 0; k < d.S; ++k) {
+        uint32_t m = pl->strip_map_h[k];
+        int f = (int)(m >> kStripFieldShift), o = (int)(m & ((1u << kStripFieldShift) - 1u));
+        StripRef r{k, f, o % d.ld + 1, o / d.ld + 1};
+        sv.push_back({tile_of(bs.fwd, r.iz, r.ix), r});
+    }
+    std::stable_sort(sv.begin(), sv.end(), [](const std::pair<int, StripRef> &x, const std::pair<int, StripRef> &y) {
+        return x.first < y.first;
+    });
+    std::vector<int> ss(bs.fwd.ntiles + 1);
+    std::vector<StripRef> sl(d.S);
+    for (int t = 0, k = 0; t <= bs.fwd.ntiles; ++t) {
+        while (k < (int)sv.size() && sv[k].first < t) ++k;
+        ss[t] = k;
+    }
+    for (int k = 0; k < d.S; ++k) sl[k] = sv[k].second;
+
+    const size_t np = (size_t)d.nxp * d.ld, ni = (size_t)d.nx * d.nz;
+    TRY(pl->alloc(&bs.rec_start, rs.size()));
+    TRY(pl->alloc(&bs.rec_list, rl.size()));
+    TRY(pl->alloc(&bs.inj_start, is.size()));
+    TRY(pl->alloc(&bs.inj_list, il.size()));
+    TRY(pl->alloc(&bs.inj_groups, ig.size()));
+    TRY(pl->alloc(&bs.strip_start, ss.size()));
+    TRY(pl->alloc(&bs.strip_list, sl.size()));
+    TRY(pl->alloc(&bs.CP, np));
+    TRY(pl->alloc(&bs.CV, np));
+    for (int k = 0; k < 2; ++k) {
+        TRY(pl->alloc(&bs.F[k], 3 * np * ns));
+        TRY(pl->alloc(&bs.Bk[k], 3 * np * ns));
+        TRY(pl->alloc(&bs.R[k], 3 * ni * ns));
+    }
+    TRY(pl->alloc(&bs.end, 3 * ni * ns));
+    TRY(pl->alloc(&bs.src, (size_t)ns * d.nt));
+    TRY(pl->alloc(&bs.acc, (size_t)ns * 3 * ni));
+    TRY(pl->alloc(&bs.out, 3 * ni));
+    HIP_TRY(hipMemcpy(bs.rec_start, rs.data(), sizeof(int) * rs.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(bs.rec_list, rl.data(), sizeof(RecRef) * rl.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(bs.inj_start, is.data(), sizeof(int) * is.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(bs.inj_list, il.data(), sizeof(RecRef) * il.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(bs.inj_groups, ig.data(), sizeof(InjGroup) * ig.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(bs.strip_start, ss.data(), sizeof(int) * ss.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(bs.strip_list, sl.data(), sizeof(StripRef) * sl.size(), hipMemcpyHostToDevice));
+    // every state buffer starts at zero; cells no step writes stay zero for good
+    for (int k = 0; k < 2; ++k) {
+        HIP_TRY(hipMemset(bs.F[k], 0, 3 * np * ns * sizeof(float)));
+        HIP_TRY(hipMemset(bs.Bk[k], 0, 3 * np * ns * sizeof(float)));
+    }
+    // shots per backward workgroup: enough workgroups to fill 256 CUs about 4 deep
+    int target = 1024;
+    int G = (int)((long long)ns * bs.bwd.ntiles / target);
+    if (G < 1) G = 1;
+    if (G > ns) G = ns;
+    if (G > 8) G = 8;
+    const char *env = getenv("SWIT_BWD_GROUP");
+    if (env && atoi(env) > 0) G = std::min(atoi(env), ns);
+    bs.G = G;
+    bs.ngroups = cdiv(ns, G);
+    int Gf = 1;
+    const char *ef = getenv("SWIT_FWD_GROUP");
+    if (ef && atoi(ef) > 0) Gf = std::min(atoi(ef), ns);
+    bs.Gf = Gf;
+    bs.ngroups_f = cdiv(ns, Gf);
+    const char *eg = getenv("SWIT_NO_GRAPH");
+    bs.use_graph = !(eg && atoi(eg) > 0);
+    HIP_TRY(hipStreamCreateWithFlags(&bs.cap, hipStreamNonBlocking));
+    bs.ready = 1;
+    return SWIT_OK;
+}
+
+static void batched_release(swit_plan *pl) {
+    BatchedState &bs = pl->bat;
+    for (auto &g : bs.gfwd)
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    if (bs.gbwd) { (void)hipGraphExecDestroy(bs.gbwd); bs.gbwd = nullptr; }
+    for (hipEvent_t e : bs.ev) (void)hipEventDestroy(e);
+    bs.ev.clear();
+    if (bs.cap) { (void)hipStreamDestroy(bs.cap); bs.cap = nullptr; }
+    bs.ready = 0;
+}
+
+static int batched_set_model(swit_plan *pl, hipStream_t st) {
+    const Dims &d = pl->d;
+    dim3 blk(64, 4), grd(cdiv(d.nzp, 64), cdiv(d.nxp, 4), 1);
+    hipLaunchKernelGGL(k_setup_cvec, grd, blk, 0, st, d, pl->K, pl->B, pl->prof, pl->bat.CP, pl->bat.CV);
+    LAUNCH_CHECK();
+    return SWIT_OK;
+}
+
+static int ensure_events(BatchedState &bs, int n) {
+    while ((int)bs.ev.size() < n) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        bs.ev.push_back(e);
+    }
+    return SWIT_OK;
+}
+
+// Enqueue the forward time loop on stream st (captured into a graph or run directly).
+static int enqueue_forward(swit_plan *pl, int save, hipStream_t st, bool timed) {
+    BatchedState &bs = pl->bat;
+    const Dims &d = pl->d;
     const int ns = pl->ns;
-    HIP_TRY(hipMemsetAsync(pl->P, 0, np * ns * sizeof(float), st));
-    HIP_TRY(hipMemsetAsync(pl->U, 0, np * ns * sizeof(float), st));
-    HIP_TRY(hipMemsetAsync(pl->W, 0, np * ns * sizeof(float), st));
-    dim3 blk(64, 4);
-    dim3 grd(cdiv(d.nzp, 64), cdiv(d.nxp, 4), ns);
-    int nrec = d.S > d.ngmax ? d.S : d.ngmax;
-    for (int it = 2; it <= d.nt; ++it) {
-        hipLaunchKernelGGL(k_fwd_p, grd, blk, 0, st, d, it, 0, pl->K, pl->damp, pl->geo, src,
-                           pl->P, pl->U, pl->W);
-        hipLaunchKernelGGL(k_fwd_v, grd, blk, 0, st, d, 0, pl->B, pl->damp, pl->P, pl->U, pl->W);
-        hipLaunchKernelGGL(k_fwd_record, dim3(cdiv(nrec, 256), 1, ns), dim3(256), 0, st, d, it, 0,
-                           pl->geo, pl->rec_fwd, pl->strip_map, pl->P, pl->U, pl->W, pl->seis_t,
-                           save ? pl->strips : nullptr);
+    const size_t np = (size_t)d.nxp * d.ld;
+    TRY(fill(bs.F[0], 3 * np * ns, st));
+    // row 0 of the seismograms and strips is zero (csg(1,:), boundary(:,1))
+    hipLaunchKernelGGL(k_zero_rows, dim3(ns), dim3(256), 0, st, pl->seis_t, (size_t)d.nt * d.ngmax, d.ngmax);
+    LAUNCH_CHECK();
+    if (save) {
+        hipLaunchKernelGGL(k_zero_rows, dim3(ns), dim3(256), 0, st, pl->strips, (size_t)d.nt * d.S, d.S);
         LAUNCH_CHECK();
+    }
+    const int nblk = bs.ngroups_f * bs.fwd.ntiles;
+    for (int it = 2; it <= d.nt; ++it) {
+        const float *in = bs.F[it % 2];          // it = 2 reads F[0]
+        float *out = bs.F[(it + 1) % 2];
+        if (timed) HIP_TRY(hipEventRecord(bs.ev[2 * (it - 2)], st));
+        hipLaunchKernelGGL(k_fwd_fused, dim3(nblk), dim3(kThreads), 0, st, d, it, bs.fwd, ns, bs.Gf, bs.CP, bs.CV,
+                           pl->geo, bs.src, in, out, bs.rec_start, bs.rec_list, pl->seis_t, bs.strip_start,
+                           bs.strip_list, save ? pl->strips : nullptr);
+        LAUNCH_CHECK();
+        if (timed) HIP_TRY(hipEventRecord(bs.ev[2 * (it - 2) + 1], st));
     }
     if (save) {
-        dim3 gi(cdiv(d.nz, 64), cdiv(d.nx, 4), ns);
-        hipLaunchKernelGGL(k_fwd_end, gi, blk, 0, st, d, 0, pl->P, pl->U, pl->W, pl->pe, pl->ue, pl->we);
+        const float *fin = bs.F[(d.nt + 1) % 2];
+        hipLaunchKernelGGL(k_end_state, dim3(cdiv(d.nz, 64), cdiv(d.nx, 4), ns), dim3(64, 4), 0, st, d, fin, bs.end);
         LAUNCH_CHECK();
     }
+    return SWIT_OK;
+}
+
+static int enqueue_adjoint(swit_plan *pl, hipStream_t st, bool timed) {
+    BatchedState &bs = pl->bat;
+    const Dims &d = pl->d;
+    const int ns = pl->ns;
+    const size_t np = (size_t)d.nxp * d.ld, ni = (size_t)d.nx * d.nz;
+    TRY(fill(bs.Bk[0], 3 * np * ns, st));
+    TRY(copyk(bs.end, bs.R[0], 3 * ni * ns, st));
+    {
+        size_t n = (size_t)bs.ngroups * 3 * ni;
+        int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_fill_d, dim3(blocks), dim3(256), 0, st, bs.acc, n);
+        LAUNCH_CHECK();
+    }
+    const int nblk = bs.ngroups * bs.bwd.ntiles;
+    for (int it = d.nt; it >= 2; --it) {
+        int k = d.nt - it;
+        const float *ri = bs.R[k % 2], *bi = bs.Bk[k % 2];
+        float *ro = bs.R[(k + 1) % 2], *bo = bs.Bk[(k + 1) % 2];
+        if (timed) HIP_TRY(hipEventRecord(bs.ev[2 * k], st));
+        hipLaunchKernelGGL(k_bwd_fused, dim3(nblk), dim3(kThreads), 0, st, d, it, bs.bwd, ns, bs.G, bs.CP, bs.CV,
+                           pl->geo, bs.src, pl->strips, ri, ro, bi, bo, bs.inj_start, bs.inj_groups,
+                           bs.inj_list, pl->sadj_t, bs.acc);
+        LAUNCH_CHECK();
+        if (timed) HIP_TRY(hipEventRecord(bs.ev[2 * k + 1], st));
+    }
+    hipLaunchKernelGGL(k_reduce_groups, dim3(cdiv((int)ni, 256)), dim3(256), 0, st, d, bs.ngroups, pl->c_int,
+                       bs.acc, bs.out, bs.out + ni, bs.out + 2 * ni);
+    LAUNCH_CHECK();
+    return SWIT_OK;
+}
+
+static int capture(swit_plan *pl, hipGraphExec_t *exec, int which, int save) {
+    BatchedState &bs = pl->bat;
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(bs.cap, hipStreamCaptureModeThreadLocal));
+    int rc = which == 0 ? enqueue_forward(pl, save, bs.cap, false) : enqueue_adjoint(pl, bs.cap, false);
+    hipError_t e = hipStreamEndCapture(bs.cap, &graph);
+    if (rc != SWIT_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return set_err(SWIT_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return set_err(SWIT_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    return SWIT_OK;
+}
+
+static int collect_profile(swit_plan *pl, int which, hipStream_t st) {
+    BatchedState &bs = pl->bat;
+    HIP_TRY(hipStreamSynchronize(st));
+    double tot = 0;
+    for (int k = 0; k < pl->d.nt - 1; ++k) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, bs.ev[2 * k], bs.ev[2 * k + 1]));
+        tot += ms;
+    }
+    bs.prof_ms[which] += tot;
+    bs.prof_n[which] += pl->d.nt - 1;
+    return SWIT_OK;
+}
+
+static int batched_forward(swit_plan *pl, const float *src, int save, hipStream_t st) {
+    BatchedState &bs = pl->bat;
+    const Dims &d = pl->d;
+    HIP_TRY(hipMemcpyAsync(bs.src, src, sizeof(float) * pl->ns * d.nt, hipMemcpyDeviceToDevice, st));
+    if (bs.profile) {
+        TRY(ensure_events(bs, 2 * (d.nt - 1)));
+        TRY(enqueue_forward(pl, save, st, true));
+        return collect_profile(pl, 0, st);
+    }
+    if (!bs.use_graph) return enqueue_forward(pl, save, st, false);
+    hipGraphExec_t &g = bs.gfwd[save ? 1 : 0];
+    if (!g) TRY(capture(pl, &g, 0, save));
+    HIP_TRY(hipGraphLaunch(g, st));
     return SWIT_OK;
 }
 
 static int batched_adjoint(swit_plan *pl, const float *src, double *grad, double *fw, double *bk,
                            hipStream_t st) {
+    BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
-    const size_t np = (size_t)d.nxp * d.ld, ni = (size_t)d.nx * d.nz;
-    const int ns = pl->ns;
-    HIP_TRY(hipMemcpyAsync(pl->Pr, pl->pe, ni * ns * sizeof(float), hipMemcpyDeviceToDevice, st));
-    HIP_TRY(hipMemcpyAsync(pl->Ur, pl->ue, ni * ns * sizeof(float), hipMemcpyDeviceToDevice, st));
-    HIP_TRY(hipMemcpyAsync(pl->Wr, pl->we, ni * ns * sizeof(float), hipMemcpyDeviceToDevice, st));
-    HIP_TRY(hipMemsetAsync(pl->Pb[0], 0, np * ns * sizeof(float), st));
-    HIP_TRY(hipMemsetAsync(pl->Pb[1], 0, np * ns * sizeof(float), st));
-    HIP_TRY(hipMemsetAsync(pl->Ub, 0, np * ns * sizeof(float), st));
-    HIP_TRY(hipMemsetAsync(pl->Wb, 0, np * ns * sizeof(float), st));
-    HIP_TRY(hipMemsetAsync(pl->dg, 0, ni * ns * sizeof(float), st));
-    HIP_TRY(hipMemsetAsync(pl->dfw, 0, ni * ns * sizeof(float), st));
-    HIP_TRY(hipMemsetAsync(pl->dbk, 0, ni * ns * sizeof(float), st));
-    dim3 blk(64, 4);
-    dim3 gp(cdiv(d.nzp, 64), cdiv(d.nxp, 4), ns);
-    dim3 gi(cdiv(d.nz, 64), cdiv(d.nx, 4), ns);
-    int cur = 0;
-    for (int it = d.nt; it >= 2; --it) {
-        hipLaunchKernelGGL(k_reco_v, gi, blk, 0, st, d, it, 0, pl->B, pl->strips, pl->Pr, pl->Ur, pl->Wr);
-        hipLaunchKernelGGL(k_reco_p, gi, blk, 0, st, d, it, 0, pl->K, pl->geo, src, pl->strips,
-                           pl->Pr, pl->Ur, pl->Wr);
-        hipLaunchKernelGGL(k_back_v, gp, blk, 0, st, d, 0, pl->K, pl->damp, pl->Pb[cur], pl->Ub, pl->Wb);
-        hipLaunchKernelGGL(k_back_p, gp, blk, 0, st, d, 0, pl->B, pl->damp, pl->Pb[cur], pl->Pb[1 - cur],
-                           pl->Ub, pl->Wb);
-        hipLaunchKernelGGL(k_back_inject, dim3(cdiv(d.ngmax, 256), 1, ns), dim3(256), 0, st, d, it, 0,
-                           pl->geo, pl->rec_inj, pl->sadj_t, pl->Pb[1 - cur]);
-        hipLaunchKernelGGL(k_back_mirror_image, gp, blk, 0, st, d, 0, pl->Pb[cur], pl->Pb[1 - cur],
-                           pl->Pr, pl->dg, pl->dfw, pl->dbk);
-        LAUNCH_CHECK();
-        cur = 1 - cur;
+    const size_t ni = (size_t)d.nx * d.nz;
+    HIP_TRY(hipMemcpyAsync(bs.src, src, sizeof(float) * pl->ns * d.nt, hipMemcpyDeviceToDevice, st));
+    if (bs.profile) {
+        TRY(ensure_events(bs, 2 * (d.nt - 1)));
+        TRY(enqueue_adjoint(pl, st, true));
+        TRY(collect_profile(pl, 1, st));
+    } else if (!bs.use_graph) {
+        TRY(enqueue_adjoint(pl, st, false));
+    } else {
+        if (!bs.gbwd) TRY(capture(pl, &bs.gbwd, 1, 0));
+        HIP_TRY(hipGraphLaunch(bs.gbwd, st));
     }
-    hipLaunchKernelGGL(k_reduce_shots, dim3(cdiv((int)ni, 256)), dim3(256), 0, st, d, ns,
-                       pl->c_int, pl->dg, pl->dfw, pl->dbk, grad, fw, bk);
-    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpyAsync(grad, bs.out, sizeof(double) * ni, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(fw, bs.out + ni, sizeof(double) * ni, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(bk, bs.out + 2 * ni, sizeof(double) * ni, hipMemcpyDeviceToDevice, st));
+    return SWIT_OK;
+}
+
+// ---- options and measurement -----------------------------------------------------------
+
+extern "C" int swit_plan_set_option(swit_plan *pl, int option, int value) {
+    g_err[0] = 0;
+    if (!pl) return set_err(SWIT_EINVAL, "plan is NULL");
+    BatchedState &bs = pl->bat;
+    switch (option) {
+    case SWIT_OPT_GRAPH:
+        bs.use_graph = value != 0;
+        return SWIT_OK;
+    case SWIT_OPT_PROFILE:
+        bs.profile = value != 0;
+        if (value) { bs.prof_ms[0] = bs.prof_ms[1] = 0; bs.prof_n[0] = bs.prof_n[1] = 0; }
+        return SWIT_OK;
+    case SWIT_OPT_BWD_GROUP:
+        if (value < 1 || value > pl->ns) return set_err(SWIT_EINVAL, "group size %d outside [1, ns]", value);
+        bs.G = value;
+        bs.ngroups = cdiv(pl->ns, value);
+        if (bs.gbwd) { (void)hipGraphExecDestroy(bs.gbwd); bs.gbwd = nullptr; }
+        return SWIT_OK;
+    case SWIT_OPT_FWD_GROUP:
+        if (value < 1 || value > pl->ns) return set_err(SWIT_EINVAL, "group size %d outside [1, ns]", value);
+        bs.Gf = value;
+        bs.ngroups_f = cdiv(pl->ns, value);
+        for (auto &gx : bs.gfwd)
+            if (gx) { (void)hipGraphExecDestroy(gx); gx = nullptr; }
+        return SWIT_OK;
+    default:
+        return set_err(SWIT_EINVAL, "unknown option %d", option);
+    }
+}
+
+extern "C" int swit_plan_kernel_time(const swit_plan *pl, int which, double *total_ms, int64_t *launches) {
+    g_err[0] = 0;
+    if (!pl || which < 0 || which > 1 || !total_ms || !launches) return set_err(SWIT_EINVAL, "bad arguments");
+    *total_ms = pl->bat.prof_ms[which];
+    *launches = pl->bat.prof_n[which];
     return SWIT_OK;
 }

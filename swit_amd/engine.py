@@ -90,6 +90,21 @@ class Plan:
     def scratch_bytes(self):
         return int(self._lib.swit_plan_scratch_bytes(self._h))
 
+    def set_option(self, option, value):
+        """option: 'graph' | 'profile' | 'bwd_group' | 'fwd_group' (batched path)."""
+        code = {"graph": _native.SWIT_OPT_GRAPH, "profile": _native.SWIT_OPT_PROFILE,
+                "bwd_group": _native.SWIT_OPT_BWD_GROUP, "fwd_group": _native.SWIT_OPT_FWD_GROUP}[option]
+        _native.check(self._lib.swit_plan_set_option(self._h, code, int(value)))
+
+    def kernel_time(self, which):
+        """(total_ms, launches) of the forward (which='forward') or backward step kernel
+        accumulated while profiling was on."""
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        _native.check(self._lib.swit_plan_kernel_time(self._h, {"forward": 0, "backward": 1}[which],
+                                                      ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
     # -- jobs ---------------------------------------------------------------------
     def set_model(self, vp, rho):
         torch = self._torch

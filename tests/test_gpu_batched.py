@@ -1,0 +1,94 @@
+"""Batched path: every mode (graph replay, direct launches, group sizes, repeated
+calls) against the FP32 oracle and against itself."""
+import numpy as np
+import pytest
+
+from helpers import integrated_sources, rel_l2
+from oracle import oracle as O
+from swit_amd import workloads
+
+
+def _case(seed=11, **kw):
+    w = workloads.small(seed=seed, **kw)
+    src = integrated_sources(w)
+    m = O.Model(w.vp_true, workloads.gardner(w.vp_true), w.dx, w.npml, "f32")
+    obs = O.forward(m, w.nt, w.dt, w.src_xz, w.rec_xz, src, save_strips=False)["seis"]
+    m = O.Model(w.vp_init, workloads.gardner(w.vp_init), w.dx, w.npml, "f32")
+    f = O.forward(m, w.nt, w.dt, w.src_xz, w.rec_xz, src)
+    adj = (f["seis"] - obs).astype(np.float32)
+    a = O.adjoint(m, w.nt, w.dt, w.src_xz, w.rec_xz, src, adj, f)
+    return w, src, adj, f, a
+
+
+def _plan(w, path="batched"):
+    import torch
+    from swit_amd.engine import Plan
+    p = Plan(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, w.src_xz, w.rec_xz, path=path)
+    p.set_model(torch.from_numpy(w.vp_init), torch.from_numpy(workloads.gardner(w.vp_init)))
+    return p
+
+
+def _run(p, src, adj):
+    import torch
+    seis = p.forward(torch.from_numpy(src), save_strips=True).cpu().numpy()
+    g = [t.cpu().numpy() for t in p.adjoint(torch.from_numpy(src), torch.from_numpy(adj))]
+    return seis, g
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("graph", [1, 0])
+def test_batched_matches_oracle(graph):
+    w, src, adj, f, a = _case()
+    p = _plan(w)
+    p.set_option("graph", graph)
+    seis, (g, fw, bk) = _run(p, src, adj)
+    p.close()
+    print(f"graph={graph}: seis max|d| {np.abs(seis - f['seis']).max():.3e}, grad relL2 {rel_l2(g, a['grad']):.3e}")
+    np.testing.assert_array_equal(seis, f["seis"])
+    assert rel_l2(g, a["grad"]) <= 1e-6
+    assert rel_l2(fw, a["illum_forw"]) <= 1e-6
+    assert rel_l2(bk, a["illum_back"]) <= 1e-6
+
+
+@pytest.mark.gpu
+def test_group_sizes_and_replays_agree():
+    w, src, adj, f, a = _case(seed=12, ns=5)
+    p = _plan(w)
+    ref = None
+    for G, Gf in ((1, 1), (2, 3), (5, 5), (3, 2)):
+        p.set_option("bwd_group", G)
+        p.set_option("fwd_group", Gf)
+        for rep in range(2):
+            seis, g = _run(p, src, adj)
+            np.testing.assert_array_equal(seis, f["seis"])
+            if ref is None:
+                ref = g
+            for x, y in zip(g, ref):
+                assert rel_l2(x, y) <= 1e-12, (G, rep)
+    assert rel_l2(ref[0], a["grad"]) <= 1e-6
+    p.close()
+
+
+@pytest.mark.gpu
+def test_adjoint_needs_saved_strips():
+    import torch
+    from swit_amd._native import SwitError
+    w, src, adj, f, a = _case(seed=13, ns=2, nt=120)
+    p = _plan(w)
+    p.forward(torch.from_numpy(src), save_strips=False)
+    with pytest.raises(SwitError):
+        p.adjoint(torch.from_numpy(src), torch.from_numpy(adj))
+    p.close()
+
+
+@pytest.mark.gpu
+def test_profile_counts_every_step():
+    w, src, adj, f, a = _case(seed=14, ns=2, nt=150)
+    p = _plan(w)
+    p.set_option("profile", 1)
+    _run(p, src, adj)
+    fm, fn = p.kernel_time("forward")
+    bm, bn = p.kernel_time("backward")
+    p.close()
+    assert fn == w.nt - 1 and bn == w.nt - 1
+    assert fm > 0 and bm > 0
