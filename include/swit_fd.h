@@ -110,7 +110,6 @@ const char *swit_last_error(void);
  * batched path). */
 #define SWIT_OPT_GRAPH 1        /* 1 (default): replay the time loop as a hipGraph    */
 #define SWIT_OPT_PROFILE 2      /* 1: time every step kernel with HIP events          */
-#define SWIT_OPT_BWD_GROUP 3    /* shots per workgroup of the backward step kernel    */
 #define SWIT_OPT_FWD_GROUP 4    /* shots per workgroup of the forward step kernel     */
 int swit_plan_set_option(swit_plan *plan, int option, int value);
 

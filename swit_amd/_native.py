@@ -37,7 +37,6 @@ EXPORTS = (
 
 SWIT_OPT_GRAPH = 1
 SWIT_OPT_PROFILE = 2
-SWIT_OPT_BWD_GROUP = 3
 SWIT_OPT_FWD_GROUP = 4
 
 

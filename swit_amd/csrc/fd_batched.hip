@@ -7,11 +7,11 @@
 // of a step always see the previous step).  Receiver samples and boundary strips of
 // the tile are written in the same launch.
 //
-// Backward step kernel: a workgroup owns one tile for a group of G shots.  Per shot
-// it runs the reverse-time reconstruction of the forward wavefield on the interior
-// window and the adjoint step on the padded grid through the same LDS tile, then adds
-// the zero-lag imaging terms into FP64 registers; one read-modify-write per step puts
-// the group's sum into its FP64 accumulators.
+// Backward step kernel: a workgroup owns one tile of one shot.  It runs the
+// reverse-time reconstruction of the forward wavefield on the interior window and the
+// adjoint step on the padded grid through the same LDS tile, then adds the zero-lag
+// imaging terms to the shot's FP32 accumulators in the reference's order; the
+// cross-shot sum is FP64 at the end of the job.
 //
 // Block -> tile order is XCD-aware: blocks are dealt round-robin to the 8 XCDs, so
 // the j-th block of XCD k takes the j-th item of a contiguous range of work items;
@@ -260,22 +260,23 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
 
 // ---------------------------------------------------------------------------------
 // Backward step it (solver.f90:230-253): reconstruction (fdcore.f90:202-289), adjoint
-// step (fdcore.f90:293-375) and imaging (solver.f90:237-239) for a group of G shots.
+// step (fdcore.f90:293-375) and imaging (solver.f90:237-239); one shot per workgroup.
 // RIn/ROut: [ns][3][nx][nz]; BIn/BOut: [ns][3][nxp][ld].
-// acc: [ngroups][3][nx][nz] FP64 (kernel, forward illumination, backward illumination).
+// acc: [ns][3][nx][nz] FP32 per-shot accumulators (kernel, forward illumination,
+// backward illumination), updated in the reference's order.
 __global__ __launch_bounds__(kThreads) void k_bwd_fused(
-    Dims d, int it, TileMap tm, int ns, int G, const float2 *__restrict__ CP, const float4 *__restrict__ CV,
+    Dims d, int it, TileMap tm, const float2 *__restrict__ CP, const float4 *__restrict__ CV,
     const ShotGeom *__restrict__ geo, const float *__restrict__ src, const float *__restrict__ strips,
     const float *__restrict__ RIn, float *__restrict__ ROut, const float *__restrict__ BIn,
     float *__restrict__ BOut, const int *__restrict__ inj_gstart, const InjGroup *__restrict__ inj_groups,
-    const RecRef *__restrict__ inj_list, const float *__restrict__ sadj_t, double *__restrict__ acc) {
+    const RecRef *__restrict__ inj_list, const float *__restrict__ sadj_t, float *__restrict__ acc) {
     __shared__ float sP[kNR];
     __shared__ float sU[kNR];
     __shared__ float sW[kNR];
     __shared__ float sQ[kNR];
 
     const int work = xcd_logical(blockIdx.x, gridDim.x);
-    const int grp = work / tm.ntiles;
+    const int s = work / tm.ntiles;
     const int t = work % tm.ntiles;
     const TileBox tb = tile_box(d, tm, t);
     const int zo = tb.z0 - kHalo, xo = tb.x0 - kHalo;
@@ -290,222 +291,201 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
     const int top0 = 3 * nx + 6 * nz;
     const float *Bc = reinterpret_cast<const float *>(CV);   // B at 4*k
     const float *Kc = reinterpret_cast<const float *>(CP);   // K at 2*k, 1-kappa_p at 2*k+1
+    const ShotGeom g = geo[s];
 
-    double ag[kCellsPerThread], af[kCellsPerThread], ab[kCellsPerThread];
+    float prn[kCellsPerThread];     // reconstructed p at it-1 on owned cells
 #pragma unroll
-    for (int c = 0; c < kCellsPerThread; ++c) ag[c] = af[c] = ab[c] = 0.0;
-
-    const int s_begin = grp * G, s_end = min(ns, s_begin + G);
-    for (int s = s_begin; s < s_end; ++s) {
-        const ShotGeom g = geo[s];
-        float prn[kCellsPerThread];     // reconstructed p at it-1 on owned cells
-#pragma unroll
-        for (int c = 0; c < kCellsPerThread; ++c) prn[c] = 0.f;
-        // ---------------- reconstruction on the interior window -----------------
-        if (has_interior) {
-            const float *rin = RIn + (size_t)s * 3 * ni;
-            const float *b = strips + ((size_t)s * d.nt + (it - 2)) * S;
-            for (int i = tid; i < kNR; i += kThreads) {
-                int jz = zo + i % kRZ - npml, jx = xo + i / kRZ - npml;
-                float p = 0.f, u = 0.f, w = 0.f;
-                if (jz >= 1 && jz <= nz && jx >= 1 && jx <= nx) {
-                    int k = (jx - 1) * nz + (jz - 1);
-                    p = rin[k];
-                    u = rin[ni + k];
-                    w = rin[2 * ni + k];
-                }
-                sP[i] = p;
-                sU[i] = u;
-                sW[i] = w;
-            }
-            __syncthreads();
-            // u', w' on rows z0-2..z0+TZ, cols x0-2..x0+TX (fdcore.f90:221-247)
-            for (int i = tid; i < kNQ; i += kThreads) {
-                int z = tb.z0 - 2 + i % kQZ, x = tb.x0 - 2 + i / kQZ;
-                int jz = z - npml, jx = x - npml;
-                if (jz < 1 || jz > nz || jx < 1 || jx > nx) continue;
-                int l = L(z, x, zo, xo);
-                if (jx == 1 || jx == nx - 1) {
-                    int jj = 3 * nx + 6 * (jz - 1);
-                    sU[l] = jx == 1 ? b[jj + 2] : b[jj + 5];
-                } else if (jx >= 2 && jx <= nx - 2 && jz >= 2 && jz <= nz - 1) {
-                    float alpha = Bc[4 * ((x - 1) * d.ld + (z - 1))];
-                    sU[l] = v_undo(sU[l], alpha, sP[l], sP[l + kRZ], sP[l + 2 * kRZ], sP[l - kRZ]);
-                }
-                if (jz == nz - 1) {
-                    sW[l] = b[3 * (jx - 1) + 2];
-                } else if (jx >= 2 && jx <= nx - 1 && jz >= 2 && jz <= nz - 2) {
-                    float alpha = Bc[4 * ((x - 1) * d.ld + (z - 1))];
-                    sW[l] = v_undo(sW[l], alpha, sP[l], sP[l + 1], sP[l + 2], sP[l - 1]);
-                }
-            }
-            __syncthreads();
-            // p' on owned tile cells (fdcore.f90:250-287)
-            float *rout = ROut + (size_t)s * 3 * ni;
-#pragma unroll
-            for (int c = 0; c < kCellsPerThread; ++c) {
-                int i = tid + c * kThreads;
-                int z = tb.z0 + i % kTZ, x = tb.x0 + i / kTZ;
-                int jz = z - npml, jx = x - npml;
-                if (z > tb.z1 || x > tb.x1 || jz < 1 || jz > nz || jx < 1 || jx > nx) continue;
-                int l = L(z, x, zo, xo);
-                float v;
-                if (jz <= 3) {
-                    v = b[top0 + 6 * (jx - 1) + (jz - 1)];
-                } else if (jx <= 2 || jx >= nx - 1) {
-                    int jj = 3 * nx + 6 * (jz - 1);
-                    v = jx == 1 ? b[jj] : jx == 2 ? b[jj + 1] : jx == nx ? b[jj + 3] : b[jj + 4];
-                } else if (jz >= nz - 1) {
-                    int jj = 3 * (jx - 1);
-                    v = jz == nz ? b[jj] : b[jj + 1];
-                } else {
-                    v = sP[l];
-                    if (jz == g.isz_i && jx == g.isx_i && g.isx_i > 2 && g.isx_i < nx - 1 && g.isz_i > 3 &&
-                        g.isz_i < nz - 1) {
-                        // the reference reads s(it+1), one past the end at it = nt (taken as 0)
-                        float sv = it + 1 <= d.nt ? src[(size_t)s * d.nt + it] : 0.0f;
-                        v = v - d.dt * sv;
-                    }
-                    float alpha = Kc[2 * ((x - 1) * d.ld + (z - 1))];
-                    v = p_undo(v, alpha, sU[l], sU[l - kRZ], sU[l + kRZ], sU[l - 2 * kRZ],
-                               sW[l], sW[l - 1], sW[l + 1], sW[l - 2]);
-                }
-                int k = (jx - 1) * nz + (jz - 1);
-                rout[k] = v;
-                rout[ni + k] = sU[l];
-                rout[2 * ni + k] = sW[l];
-                prn[c] = v;
-            }
-            __syncthreads();
-        }
-        // ---------------- adjoint step on the padded grid ------------------------
-        const float *bin = BIn + (size_t)s * 3 * np;
+    for (int c = 0; c < kCellsPerThread; ++c) prn[c] = 0.f;
+    // ---------------- reconstruction on the interior window -----------------
+    if (has_interior) {
+        const float *rin = RIn + (size_t)s * 3 * ni;
+        const float *b = strips + ((size_t)s * d.nt + (it - 2)) * S;
         for (int i = tid; i < kNR; i += kThreads) {
-            int z = zo + i % kRZ, x = xo + i / kRZ;
+            int jz = zo + i % kRZ - npml, jx = xo + i / kRZ - npml;
             float p = 0.f, u = 0.f, w = 0.f;
-            if (z >= 1 && z <= d.nzp && x >= 1 && x <= d.nxp) {
-                int k = (x - 1) * d.ld + (z - 1);
-                p = bin[k];
-                u = bin[np + k];
-                w = bin[2 * np + k];
+            if (jz >= 1 && jz <= nz && jx >= 1 && jx <= nx) {
+                int k = (jx - 1) * nz + (jz - 1);
+                p = rin[k];
+                u = rin[ni + k];
+                w = rin[2 * ni + k];
             }
             sP[i] = p;
             sU[i] = u;
             sW[i] = w;
         }
         __syncthreads();
-        // u', w' with alpha = den c^2 dt/dx on the region (fdcore.f90:310-326)
+        // u', w' on rows z0-2..z0+TZ, cols x0-2..x0+TX (fdcore.f90:221-247)
         for (int i = tid; i < kNQ; i += kThreads) {
             int z = tb.z0 - 2 + i % kQZ, x = tb.x0 - 2 + i / kQZ;
-            bool ur = z >= 2 && z <= d.nzp - 1 && x >= 2 && x <= d.nxp - 2;
-            bool wr = z >= 2 && z <= d.nzp - 2 && x >= 2 && x <= d.nxp - 1;
-            if (!(ur || wr)) continue;
-            int l = L(z, x, zo, xo);
-            int k = (x - 1) * d.ld + (z - 1);
-            float alpha = Kc[2 * k];
-            float4 cv = CV[k];
-            if (ur) sU[l] = v_update_f(sU[l], alpha, cv.y, sP[l], sP[l + kRZ], sP[l + 2 * kRZ], sP[l - kRZ]);
-            if (wr) sW[l] = v_update_f(sW[l], alpha, cv.z, sP[l], sP[l + 1], sP[l + 2], sP[l - 1]);
-        }
-        __syncthreads();
-        // w mirror (fdcore.f90:329-333), columns 2..nxp-1 of the region
-        if (d.free_surface == 1 && fs - 1 >= tb.z0 - 2 && fs <= tb.z0 + kTZ) {
-            for (int x = tb.x0 - 2 + tid; x <= tb.x0 + kTX; x += kThreads)
-                if (x >= 2 && x <= d.nxp - 1) sW[L(fs - 1, x, zo, xo)] = sW[L(fs, x, zo, xo)];
-        }
-        __syncthreads();
-        // p' with alpha = dt/(den dx) on the tile into sQ (fdcore.f90:336-345)
-#pragma unroll
-        for (int c = 0; c < kCellsPerThread; ++c) {
-            int i = tid + c * kThreads;
-            int z = tb.z0 + i % kTZ, x = tb.x0 + i / kTZ;
-            if (z > tb.z1 || x > tb.x1) continue;
-            int l = L(z, x, zo, xo);
-            float v = sP[l];
-            if (z >= 3 && z <= d.nzp - 2 && x >= 3 && x <= d.nxp - 2) {
-                int k = (x - 1) * d.ld + (z - 1);
-                float alpha = Bc[4 * k];
-                float fac = Kc[2 * k + 1];
-                v = p_update_f(v, alpha, fac, sU[l], sU[l - kRZ], sU[l + kRZ], sU[l - 2 * kRZ],
-                               sW[l], sW[l - 1], sW[l + 1], sW[l - 2]);
-            }
-            sQ[l] = v;
-        }
-        __syncthreads();
-        // adjoint-source injection (fdcore.f90:348-367): one thread per cell, receivers
-        // sharing a cell added in receiver order
-        {
-            int g0 = inj_gstart[(size_t)s * (tm.ntiles + 1) + t];
-            int g1 = inj_gstart[(size_t)s * (tm.ntiles + 1) + t + 1];
-            const RecRef *el = inj_list + (size_t)s * d.ngmax;
-            const float *sa = sadj_t + ((size_t)s * d.nt + (it - 1)) * d.ngmax;
-            for (int q = g0 + tid; q < g1; q += kThreads) {
-                InjGroup gr = inj_groups[(size_t)s * d.ngmax + q];
-                RecRef r0 = el[gr.e0];
-                int l = L(r0.iz, r0.ix, zo, xo);
-                float v = sQ[l];
-                for (int e = gr.e0; e < gr.e1; ++e) v = v + d.dt * sa[el[e].idx];
-                sQ[l] = v;
-            }
-        }
-        __syncthreads();
-        // unconditional p mirror (fdcore.f90:370-373): rows fs-1 and fs+1 share a tile
-        if (fs - 1 >= tb.z0 && fs + 1 <= tb.z1) {
-            for (int x = tb.x0 + tid; x <= tb.x1; x += kThreads) sQ[L(fs - 1, x, zo, xo)] = -sQ[L(fs + 1, x, zo, xo)];
-        }
-        __syncthreads();
-        // write back, imaging on owned cells
-        float *bout = BOut + (size_t)s * 3 * np;
-#pragma unroll
-        for (int c = 0; c < kCellsPerThread; ++c) {
-            int i = tid + c * kThreads;
-            int z = tb.z0 + i % kTZ, x = tb.x0 + i / kTZ;
-            if (z > tb.z1 || x > tb.x1) continue;
-            int l = L(z, x, zo, xo), k = (x - 1) * d.ld + (z - 1);
-            float pb = sQ[l];
-            bout[k] = pb;
-            bout[np + k] = sU[l];
-            bout[2 * np + k] = sW[l];
             int jz = z - npml, jx = x - npml;
-            if (jz >= 1 && jz <= nz && jx >= 1 && jx <= nx) {
-                float pr = prn[c];
-                ag[c] += (double)(pr * (sP[l] - pb));
-                af[c] += (double)(pr * pr);
-                ab[c] += (double)(pb * pb);
+            if (jz < 1 || jz > nz || jx < 1 || jx > nx) continue;
+            int l = L(z, x, zo, xo);
+            if (jx == 1 || jx == nx - 1) {
+                int jj = 3 * nx + 6 * (jz - 1);
+                sU[l] = jx == 1 ? b[jj + 2] : b[jj + 5];
+            } else if (jx >= 2 && jx <= nx - 2 && jz >= 2 && jz <= nz - 1) {
+                float alpha = Bc[4 * ((x - 1) * d.ld + (z - 1))];
+                sU[l] = v_undo(sU[l], alpha, sP[l], sP[l + kRZ], sP[l + 2 * kRZ], sP[l - kRZ]);
+            }
+            if (jz == nz - 1) {
+                sW[l] = b[3 * (jx - 1) + 2];
+            } else if (jx >= 2 && jx <= nx - 1 && jz >= 2 && jz <= nz - 2) {
+                float alpha = Bc[4 * ((x - 1) * d.ld + (z - 1))];
+                sW[l] = v_undo(sW[l], alpha, sP[l], sP[l + 1], sP[l + 2], sP[l - 1]);
             }
         }
         __syncthreads();
-    }
-    // accumulate the group's contribution of this step
-    if (has_interior) {
-        double *a = acc + (size_t)grp * 3 * ni;
+        // p' on owned tile cells (fdcore.f90:250-287)
+        float *rout = ROut + (size_t)s * 3 * ni;
 #pragma unroll
         for (int c = 0; c < kCellsPerThread; ++c) {
             int i = tid + c * kThreads;
             int z = tb.z0 + i % kTZ, x = tb.x0 + i / kTZ;
             int jz = z - npml, jx = x - npml;
             if (z > tb.z1 || x > tb.x1 || jz < 1 || jz > nz || jx < 1 || jx > nx) continue;
+            int l = L(z, x, zo, xo);
+            float v;
+            if (jz <= 3) {
+                v = b[top0 + 6 * (jx - 1) + (jz - 1)];
+            } else if (jx <= 2 || jx >= nx - 1) {
+                int jj = 3 * nx + 6 * (jz - 1);
+                v = jx == 1 ? b[jj] : jx == 2 ? b[jj + 1] : jx == nx ? b[jj + 3] : b[jj + 4];
+            } else if (jz >= nz - 1) {
+                int jj = 3 * (jx - 1);
+                v = jz == nz ? b[jj] : b[jj + 1];
+            } else {
+                v = sP[l];
+                if (jz == g.isz_i && jx == g.isx_i && g.isx_i > 2 && g.isx_i < nx - 1 && g.isz_i > 3 &&
+                    g.isz_i < nz - 1) {
+                    // the reference reads s(it+1), one past the end at it = nt (taken as 0)
+                    float sv = it + 1 <= d.nt ? src[(size_t)s * d.nt + it] : 0.0f;
+                    v = v - d.dt * sv;
+                }
+                float alpha = Kc[2 * ((x - 1) * d.ld + (z - 1))];
+                v = p_undo(v, alpha, sU[l], sU[l - kRZ], sU[l + kRZ], sU[l - 2 * kRZ],
+                           sW[l], sW[l - 1], sW[l + 1], sW[l - 2]);
+            }
             int k = (jx - 1) * nz + (jz - 1);
-            a[k] += ag[c];
-            a[ni + k] += af[c];
-            a[2 * ni + k] += ab[c];
+            rout[k] = v;
+            rout[ni + k] = sU[l];
+            rout[2 * ni + k] = sW[l];
+            prn[c] = v;
+        }
+        __syncthreads();
+    }
+    // ---------------- adjoint step on the padded grid ------------------------
+    const float *bin = BIn + (size_t)s * 3 * np;
+    for (int i = tid; i < kNR; i += kThreads) {
+        int z = zo + i % kRZ, x = xo + i / kRZ;
+        float p = 0.f, u = 0.f, w = 0.f;
+        if (z >= 1 && z <= d.nzp && x >= 1 && x <= d.nxp) {
+            int k = (x - 1) * d.ld + (z - 1);
+            p = bin[k];
+            u = bin[np + k];
+            w = bin[2 * np + k];
+        }
+        sP[i] = p;
+        sU[i] = u;
+        sW[i] = w;
+    }
+    __syncthreads();
+    // u', w' with alpha = den c^2 dt/dx on the region (fdcore.f90:310-326)
+    for (int i = tid; i < kNQ; i += kThreads) {
+        int z = tb.z0 - 2 + i % kQZ, x = tb.x0 - 2 + i / kQZ;
+        bool ur = z >= 2 && z <= d.nzp - 1 && x >= 2 && x <= d.nxp - 2;
+        bool wr = z >= 2 && z <= d.nzp - 2 && x >= 2 && x <= d.nxp - 1;
+        if (!(ur || wr)) continue;
+        int l = L(z, x, zo, xo);
+        int k = (x - 1) * d.ld + (z - 1);
+        float alpha = Kc[2 * k];
+        float4 cv = CV[k];
+        if (ur) sU[l] = v_update_f(sU[l], alpha, cv.y, sP[l], sP[l + kRZ], sP[l + 2 * kRZ], sP[l - kRZ]);
+        if (wr) sW[l] = v_update_f(sW[l], alpha, cv.z, sP[l], sP[l + 1], sP[l + 2], sP[l - 1]);
+    }
+    __syncthreads();
+    // w mirror (fdcore.f90:329-333), columns 2..nxp-1 of the region
+    if (d.free_surface == 1 && fs - 1 >= tb.z0 - 2 && fs <= tb.z0 + kTZ) {
+        for (int x = tb.x0 - 2 + tid; x <= tb.x0 + kTX; x += kThreads)
+            if (x >= 2 && x <= d.nxp - 1) sW[L(fs - 1, x, zo, xo)] = sW[L(fs, x, zo, xo)];
+    }
+    __syncthreads();
+    // p' with alpha = dt/(den dx) on the tile into sQ (fdcore.f90:336-345)
+#pragma unroll
+    for (int c = 0; c < kCellsPerThread; ++c) {
+        int i = tid + c * kThreads;
+        int z = tb.z0 + i % kTZ, x = tb.x0 + i / kTZ;
+        if (z > tb.z1 || x > tb.x1) continue;
+        int l = L(z, x, zo, xo);
+        float v = sP[l];
+        if (z >= 3 && z <= d.nzp - 2 && x >= 3 && x <= d.nxp - 2) {
+            int k = (x - 1) * d.ld + (z - 1);
+            float alpha = Bc[4 * k];
+            float fac = Kc[2 * k + 1];
+            v = p_update_f(v, alpha, fac, sU[l], sU[l - kRZ], sU[l + kRZ], sU[l - 2 * kRZ],
+                           sW[l], sW[l - 1], sW[l + 1], sW[l - 2]);
+        }
+        sQ[l] = v;
+    }
+    __syncthreads();
+    // adjoint-source injection (fdcore.f90:348-367): one thread per cell, receivers
+    // sharing a cell added in receiver order
+    {
+        int g0 = inj_gstart[(size_t)s * (tm.ntiles + 1) + t];
+        int g1 = inj_gstart[(size_t)s * (tm.ntiles + 1) + t + 1];
+        const RecRef *el = inj_list + (size_t)s * d.ngmax;
+        const float *sa = sadj_t + ((size_t)s * d.nt + (it - 1)) * d.ngmax;
+        for (int q = g0 + tid; q < g1; q += kThreads) {
+            InjGroup gr = inj_groups[(size_t)s * d.ngmax + q];
+            RecRef r0 = el[gr.e0];
+            int l = L(r0.iz, r0.ix, zo, xo);
+            float v = sQ[l];
+            for (int e = gr.e0; e < gr.e1; ++e) v = v + d.dt * sa[el[e].idx];
+            sQ[l] = v;
+        }
+    }
+    __syncthreads();
+    // unconditional p mirror (fdcore.f90:370-373): rows fs-1 and fs+1 share a tile
+    if (fs - 1 >= tb.z0 && fs + 1 <= tb.z1) {
+        for (int x = tb.x0 + tid; x <= tb.x1; x += kThreads) sQ[L(fs - 1, x, zo, xo)] = -sQ[L(fs + 1, x, zo, xo)];
+    }
+    __syncthreads();
+    // write back; imaging dg += p_rec (p0 - p), dfw += p_rec^2, dbk += p^2
+    float *bout = BOut + (size_t)s * 3 * np;
+    float *a = acc + (size_t)s * 3 * ni;
+#pragma unroll
+    for (int c = 0; c < kCellsPerThread; ++c) {
+        int i = tid + c * kThreads;
+        int z = tb.z0 + i % kTZ, x = tb.x0 + i / kTZ;
+        if (z > tb.z1 || x > tb.x1) continue;
+        int l = L(z, x, zo, xo), k = (x - 1) * d.ld + (z - 1);
+        float pb = sQ[l];
+        bout[k] = pb;
+        bout[np + k] = sU[l];
+        bout[2 * np + k] = sW[l];
+        int jz = z - npml, jx = x - npml;
+        if (jz >= 1 && jz <= nz && jx >= 1 && jx <= nx) {
+            int ki = (jx - 1) * nz + (jz - 1);
+            float pr = prn[c];
+            a[ki] = a[ki] + pr * (sP[l] - pb);
+            a[ni + ki] = a[ni + ki] + pr * pr;
+            a[2 * ni + ki] = a[2 * ni + ki] + pb * pb;
         }
     }
 }
 
-// Sum of the group accumulators (FP64), g = -2 g / c (solver.f90:263-270).
-__global__ void k_reduce_groups(Dims d, int ngroups, const float *__restrict__ vp,
-                                const double *__restrict__ acc, double *__restrict__ grad,
-                                double *__restrict__ fw, double *__restrict__ bk) {
+// Cross-shot FP64 sum of the per-shot kernels and illuminations
+// (solver.f90:263-265) and g = -2 g / c (solver.f90:270).  acc: [ns][3][nx][nz].
+__global__ void k_reduce_acc(Dims d, int ns, const float *__restrict__ vp, const float *__restrict__ acc,
+                             double *__restrict__ grad, double *__restrict__ fw, double *__restrict__ bk) {
     size_t ni = (size_t)d.nx * d.nz;
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ni) return;
     double g = 0.0, f = 0.0, b = 0.0;
-    for (int q = 0; q < ngroups; ++q) {
-        const double *a = acc + (size_t)q * 3 * ni;
-        g += a[k];
-        f += a[ni + k];
-        b += a[2 * ni + k];
+    for (int s = 0; s < ns; ++s) {
+        const float *a = acc + (size_t)s * 3 * ni;
+        g += (double)a[k];
+        f += (double)a[ni + k];
+        b += (double)a[2 * ni + k];
     }
     grad[k] = -2.0 * g / (double)vp[k];
     fw[k] = f;
@@ -530,8 +510,6 @@ __global__ void k_end_state(Dims d, const float *__restrict__ S, float *__restri
 // Per-plan state of the batched path.
 struct BatchedState {
     int ready = 0;
-    int G = 1;                   // shots per backward workgroup
-    int ngroups = 0;
     int Gf = 1;                  // shots per forward workgroup
     int ngroups_f = 0;
     TileMap fwd{}, bwd{};
@@ -549,7 +527,7 @@ struct BatchedState {
     float *R[2] = {nullptr, nullptr};   // reconstruction state [ns][3][nx][nz]
     float *end = nullptr;        // saved final forward state [ns][3][nx][nz]
     float *src = nullptr;        // plan-owned copy of the source traces
-    double *acc = nullptr;       // [ns][3][nx][nz] (first ngroups used)
+    float *acc = nullptr;        // [ns][3][nx][nz] per-shot FP32 accumulators
     double *out = nullptr;       // [3][nx][nz]
     hipStream_t cap = nullptr;   // capture stream
     hipGraphExec_t gfwd[2] = {nullptr, nullptr};

@@ -536,16 +536,6 @@ static int batched_init(swit_plan *pl) {
         HIP_TRY(hipMemset(bs.F[k], 0, 3 * np * ns * sizeof(float)));
         HIP_TRY(hipMemset(bs.Bk[k], 0, 3 * np * ns * sizeof(float)));
     }
-    // shots per backward workgroup: enough workgroups to fill 256 CUs about 4 deep
-    int target = 1024;
-    int G = (int)((long long)ns * bs.bwd.ntiles / target);
-    if (G < 1) G = 1;
-    if (G > ns) G = ns;
-    if (G > 8) G = 8;
-    const char *env = getenv("SWIT_BWD_GROUP");
-    if (env && atoi(env) > 0) G = std::min(atoi(env), ns);
-    bs.G = G;
-    bs.ngroups = cdiv(ns, G);
     int Gf = 1;
     const char *ef = getenv("SWIT_FWD_GROUP");
     if (ef && atoi(ef) > 0) Gf = std::min(atoi(ef), ns);
@@ -626,26 +616,21 @@ static int enqueue_adjoint(swit_plan *pl, hipStream_t st, bool timed) {
     const size_t np = (size_t)d.nxp * d.ld, ni = (size_t)d.nx * d.nz;
     TRY(fill(bs.Bk[0], 3 * np * ns, st));
     TRY(copyk(bs.end, bs.R[0], 3 * ni * ns, st));
-    {
-        size_t n = (size_t)bs.ngroups * 3 * ni;
-        int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
-        hipLaunchKernelGGL(k_fill_d, dim3(blocks), dim3(256), 0, st, bs.acc, n);
-        LAUNCH_CHECK();
-    }
-    const int nblk = bs.ngroups * bs.bwd.ntiles;
+    TRY(fill(bs.acc, 3 * ni * ns, st));
+    const int nblk = ns * bs.bwd.ntiles;
     for (int it = d.nt; it >= 2; --it) {
         int k = d.nt - it;
         const float *ri = bs.R[k % 2], *bi = bs.Bk[k % 2];
         float *ro = bs.R[(k + 1) % 2], *bo = bs.Bk[(k + 1) % 2];
         if (timed) HIP_TRY(hipEventRecord(bs.ev[2 * k], st));
-        hipLaunchKernelGGL(k_bwd_fused, dim3(nblk), dim3(kThreads), 0, st, d, it, bs.bwd, ns, bs.G, bs.CP, bs.CV,
+        hipLaunchKernelGGL(k_bwd_fused, dim3(nblk), dim3(kThreads), 0, st, d, it, bs.bwd, bs.CP, bs.CV,
                            pl->geo, bs.src, pl->strips, ri, ro, bi, bo, bs.inj_start, bs.inj_groups,
                            bs.inj_list, pl->sadj_t, bs.acc);
         LAUNCH_CHECK();
         if (timed) HIP_TRY(hipEventRecord(bs.ev[2 * k + 1], st));
     }
-    hipLaunchKernelGGL(k_reduce_groups, dim3(cdiv((int)ni, 256)), dim3(256), 0, st, d, bs.ngroups, pl->c_int,
-                       bs.acc, bs.out, bs.out + ni, bs.out + 2 * ni);
+    hipLaunchKernelGGL(k_reduce_acc, dim3(cdiv((int)ni, 256)), dim3(256), 0, st, d, ns, pl->c_int, bs.acc,
+                       bs.out, bs.out + ni, bs.out + 2 * ni);
     LAUNCH_CHECK();
     return SWIT_OK;
 }
@@ -732,12 +717,6 @@ extern "C" int swit_plan_set_option(swit_plan *pl, int option, int value) {
     case SWIT_OPT_PROFILE:
         bs.profile = value != 0;
         if (value) { bs.prof_ms[0] = bs.prof_ms[1] = 0; bs.prof_n[0] = bs.prof_n[1] = 0; }
-        return SWIT_OK;
-    case SWIT_OPT_BWD_GROUP:
-        if (value < 1 || value > pl->ns) return set_err(SWIT_EINVAL, "group size %d outside [1, ns]", value);
-        bs.G = value;
-        bs.ngroups = cdiv(pl->ns, value);
-        if (bs.gbwd) { (void)hipGraphExecDestroy(bs.gbwd); bs.gbwd = nullptr; }
         return SWIT_OK;
     case SWIT_OPT_FWD_GROUP:
         if (value < 1 || value > pl->ns) return set_err(SWIT_EINVAL, "group size %d outside [1, ns]", value);

@@ -91,9 +91,9 @@ class Plan:
         return int(self._lib.swit_plan_scratch_bytes(self._h))
 
     def set_option(self, option, value):
-        """option: 'graph' | 'profile' | 'bwd_group' | 'fwd_group' (batched path)."""
+        """option: 'graph' | 'profile' | 'fwd_group' (batched path)."""
         code = {"graph": _native.SWIT_OPT_GRAPH, "profile": _native.SWIT_OPT_PROFILE,
-                "bwd_group": _native.SWIT_OPT_BWD_GROUP, "fwd_group": _native.SWIT_OPT_FWD_GROUP}[option]
+                "fwd_group": _native.SWIT_OPT_FWD_GROUP}[option]
         _native.check(self._lib.swit_plan_set_option(self._h, code, int(value)))
 
     def kernel_time(self, which):

@@ -45,9 +45,11 @@ def test_batched_matches_oracle(graph):
     p.close()
     print(f"graph={graph}: seis max|d| {np.abs(seis - f['seis']).max():.3e}, grad relL2 {rel_l2(g, a['grad']):.3e}")
     np.testing.assert_array_equal(seis, f["seis"])
-    assert rel_l2(g, a["grad"]) <= 1e-6
-    assert rel_l2(fw, a["illum_forw"]) <= 1e-6
-    assert rel_l2(bk, a["illum_back"]) <= 1e-6
+    # per-shot FP32 imaging in the reference's order: bit-identical per-shot kernels,
+    # FP64 cross-shot sums in the same shot order as the oracle
+    assert rel_l2(g, a["grad"]) <= 1e-12
+    assert rel_l2(fw, a["illum_forw"]) <= 1e-12
+    assert rel_l2(bk, a["illum_back"]) <= 1e-12
 
 
 @pytest.mark.gpu
@@ -55,17 +57,16 @@ def test_group_sizes_and_replays_agree():
     w, src, adj, f, a = _case(seed=12, ns=5)
     p = _plan(w)
     ref = None
-    for G, Gf in ((1, 1), (2, 3), (5, 5), (3, 2)):
-        p.set_option("bwd_group", G)
-        p.set_option("fwd_group", Gf)
+    for G in (1, 3, 5, 2):
+        p.set_option("fwd_group", G)
         for rep in range(2):
             seis, g = _run(p, src, adj)
             np.testing.assert_array_equal(seis, f["seis"])
             if ref is None:
                 ref = g
             for x, y in zip(g, ref):
-                assert rel_l2(x, y) <= 1e-12, (G, rep)
-    assert rel_l2(ref[0], a["grad"]) <= 1e-6
+                assert rel_l2(x, y) == 0.0, (G, rep)
+    assert rel_l2(ref[0], a["grad"]) <= 1e-12
     p.close()
 
 
