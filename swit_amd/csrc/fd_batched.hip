@@ -11,12 +11,13 @@
 // reverse-time reconstruction of the forward wavefield on the interior window and the
 // adjoint step on the padded grid through the same LDS tile, then adds the zero-lag
 // imaging terms to the shot's FP32 accumulators in the reference's order; the
-// cross-shot sum is FP64 at the end of the job.
+// cross-shot sum is FP64 at the end of the job.  (Processing several shots per
+// workgroup here needs about twice the registers and runs slower.)
 //
 // Block -> tile order is XCD-aware: blocks are dealt round-robin to the 8 XCDs, so
 // the j-th block of XCD k takes the j-th item of a contiguous range of work items;
-// items are ordered z-fastest within a shot group so neighbouring tiles (which share
-// halos) run on the same XCD, close in time, and hit the same L2.
+// items walk a group of tile columns for every shot before moving on, so the group's
+// coefficients and shared halos stay in that XCD's L2 (see work_item).
 //
 // Memory layout (device): state buffers [ns][3][nxp][ld] (p, u, w of a shot
 // contiguous), reconstruction buffers [ns][3][nx][nz]; coefficients per padded cell
@@ -28,25 +29,48 @@
 
 namespace swit {
 
-constexpr int kTZ = 64;          // tile rows (iz, the contiguous axis)
-constexpr int kTX = 16;          // tile columns (ix)
+constexpr int kTX = 16;          // output columns of a tile (both step kernels)
 constexpr int kHalo = 3;
+constexpr int kThreads = 512;
+constexpr int kWaves = kThreads / 64;
+constexpr int kNumXcd = 8;
+
+// Forward step kernel tile: kTZ x kTX outputs, LDS window (kTZ+6) x (kTX+6).
+namespace fk {
+constexpr int kTZ = 64;
 constexpr int kRZ = kTZ + 2 * kHalo;
 constexpr int kRX = kTX + 2 * kHalo;
 constexpr int kNR = kRZ * kRX;
-constexpr int kThreads = 512;
 constexpr int kCellsPerThread = kTZ * kTX / kThreads;
-constexpr int kNumXcd = 8;
-// Region of new values the tile's stencils read: (TZ+3) x (TX+3) cells.
+// region of new p the velocity stencils read: (kTZ+3) x (kTX+3) cells
 constexpr int kQZ = kTZ + 3;
 constexpr int kQX = kTX + 3;
 constexpr int kNQ = kQZ * kQX;
+constexpr int kLPT = (kNR + kThreads - 1) / kThreads;   // staged cells per thread
+constexpr int kQPT = (kNQ + kThreads - 1) / kThreads;   // region cells per thread
+// LDS index of padded cell (z, x) in a window whose origin is (zo, xo).
+__device__ __forceinline__ int L(int z, int x, int zo, int xo) { return (x - xo) * kRZ + (z - zo); }
+}  // namespace fk
+
+// Backward step kernel tile: kTZ x kTX outputs.  Its LDS window is one wave tall
+// (kZL = 64 rows: the outputs plus a 3-row halo above and below) and kCX columns wide;
+// lane l of every wave owns window row l, so each row predicate is evaluated once per
+// thread and every column loop is uniform across the wave.
+namespace bk {
+constexpr int kZL = 64;
+constexpr int kTZ = kZL - 2 * kHalo;   // 58 output rows
+constexpr int kCX = kTX + 2 * kHalo;   // 22 window columns
+constexpr int kNL = kZL * kCX;         // floats per LDS array
+// LDS index of window column c, row (lane) l.
+__device__ __forceinline__ int LW(int c, int l) { return c * kZL + l; }
+}  // namespace bk
 
 // Tiling of the padded grid: tile (zt, xt) covers rows
-// [max(ztop + zt*TZ, 1), min(ztop + (zt+1)*TZ - 1, nzp)] and columns
+// [max(ztop + zt*th, 1), min(ztop + (zt+1)*th - 1, nzp)] and columns
 // [1 + xt*TX, min((xt+1)*TX, nxp)].
 struct TileMap {
     int ztop;
+    int th;         // tile height (rows)
     int nzt, nxt;
     int ntiles;     // nzt * nxt
 };
@@ -58,8 +82,8 @@ struct TileBox {
 __device__ __forceinline__ TileBox tile_box(const Dims &d, const TileMap &tm, int t) {
     int zt = t % tm.nzt, xt = t / tm.nzt;
     TileBox b;
-    b.z0 = max(tm.ztop + zt * kTZ, 1);
-    b.z1 = min(tm.ztop + (zt + 1) * kTZ - 1, d.nzp);
+    b.z0 = max(tm.ztop + zt * tm.th, 1);
+    b.z1 = min(tm.ztop + (zt + 1) * tm.th - 1, d.nzp);
     b.x0 = 1 + xt * kTX;
     b.x1 = min((xt + 1) * kTX, d.nxp);
     return b;
@@ -70,6 +94,21 @@ __device__ __forceinline__ int xcd_logical(int b, int nb) {
     int q = nb / kNumXcd, r = nb % kNumXcd;
     int k = b % kNumXcd, j = b / kNumXcd;
     return k * q + min(k, r) + j;
+}
+
+// Logical work item -> (shot group, tile).  Items run through groups of kColGroup
+// tile columns; inside a group every shot group visits all of the group's tiles
+// before the next one starts.  A group's coefficients (shared by all shots) and the
+// halos its tiles share stay in the XCD's L2 while the shots stream through it.
+constexpr int kColGroup = 4;
+__device__ __forceinline__ void work_item(int w, int nshot, const TileMap &tm, int &s, int &t) {
+    const int per = kColGroup * tm.nzt;              // tiles of a full column group
+    const int gq = w / (per * nshot);
+    const int rem = w - gq * per * nshot;
+    const int cols = min(kColGroup, tm.nxt - gq * kColGroup);
+    const int tiles = cols * tm.nzt;
+    s = rem / tiles;
+    t = gq * per + rem % tiles;
 }
 
 // Strip slot of a tile: slot index and source cell.
@@ -91,8 +130,6 @@ struct InjGroup {
     int e0, e1;     // entry range in the shot's injection list
 };
 
-// LDS index of padded cell (z, x) in a region whose origin is (zo, xo).
-__device__ __forceinline__ int L(int z, int x, int zo, int xo) { return (x - xo) * kRZ + (z - zo); }
 
 // ---- small helpers (kernels only, so captured graphs hold no memset nodes) -------
 __global__ void k_fill(float *__restrict__ p, size_t n, float v) {
@@ -139,57 +176,105 @@ __global__ void k_setup_cvec(Dims d, const float *__restrict__ K, const float *_
 
 // ---------------------------------------------------------------------------------
 // Forward step it (fdcore.f90:75-160) for a group of G shots per workgroup.
-// In/Out: [ns][3][nxp][ld].
+// In/Out: [ns][3][nxp][ld].  LDS window origin (zo, xo) = (z0-3, x0-3).
 __global__ __launch_bounds__(kThreads) void k_fwd_fused(
     Dims d, int it, TileMap tm, int ns, int G, const float2 *__restrict__ CP, const float4 *__restrict__ CV,
     const ShotGeom *__restrict__ geo, const float *__restrict__ src, const float *__restrict__ In,
     float *__restrict__ Out, const int *__restrict__ rec_start, const RecRef *__restrict__ rec_list,
     float *__restrict__ seis_t, const int *__restrict__ strip_start, const StripRef *__restrict__ strip_list,
     float *__restrict__ strips) {
+    using namespace fk;
     __shared__ float sP[kNR];
     __shared__ float sU[kNR];
     __shared__ float sW[kNR];
     __shared__ float sQ[kNR];     // new p
 
-    const int work = xcd_logical(blockIdx.x, gridDim.x);
-    const int grp = work / tm.ntiles;
-    const int t = work % tm.ntiles;
+    int grp, t;
+    work_item(xcd_logical(blockIdx.x, gridDim.x), (ns + G - 1) / G, tm, grp, t);
     const TileBox tb = tile_box(d, tm, t);
     const int zo = tb.z0 - kHalo, xo = tb.x0 - kHalo;
     const int tid = threadIdx.x;
     const int fs = d.fs;
     const int np = d.nxp * d.ld;
-
     const int s_begin = grp * G, s_end = min(ns, s_begin + G);
+
+    // staged fields of the first shot
+    float lp[kLPT], lu[kLPT], lw[kLPT];
+    auto stage_load = [&](int s) {
+        const float *pin = In + (size_t)s * 3 * np;
+#pragma unroll
+        for (int j = 0; j < kLPT; ++j) {
+            int i = tid + j * kThreads;
+            int z = zo + i % kRZ, x = xo + i / kRZ;
+            lp[j] = lu[j] = lw[j] = 0.f;
+            if (i < kNR && z >= 1 && z <= d.nzp && x >= 1 && x <= d.nxp) {
+                const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
+                lp[j] = pin[k];
+                lu[j] = pin[np + k];
+                lw[j] = pin[2 * np + k];
+            }
+        }
+    };
+    stage_load(s_begin);
+    // coefficients of this thread's region and tile cells, shared by every shot
+    float2 cp[kQPT];
+    unsigned qgrid = 0, qupd = 0;
+#pragma unroll
+    for (int c = 0; c < kQPT; ++c) {
+        int i = tid + c * kThreads;
+        int z = tb.z0 - 1 + i % kQZ, x = tb.x0 - 1 + i / kQZ;
+        cp[c] = make_float2(0.f, 0.f);
+        if (i < kNQ && z >= 1 && z <= d.nzp && x >= 1 && x <= d.nxp) {
+            qgrid |= 1u << c;
+            if (z >= fs + 1 && z <= d.nzp - 2 && x >= 3 && x <= d.nxp - 2) {
+                qupd |= 1u << c;
+                cp[c] = CP[(unsigned)((x - 1) * d.ld + (z - 1))];
+            }
+        }
+    }
+    float4 cv[kCellsPerThread];
+    unsigned tur = 0, twr = 0, tin = 0;
+#pragma unroll
+    for (int c = 0; c < kCellsPerThread; ++c) {
+        int i = tid + c * kThreads;
+        int z = tb.z0 + i % kTZ, x = tb.x0 + i / kTZ;
+        cv[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (z <= tb.z1 && x <= tb.x1) {
+            tin |= 1u << c;
+            bool ur = z >= fs && z <= d.nzp - 1 && x >= 2 && x <= d.nxp - 2;
+            bool wr = z >= fs && z <= d.nzp - 2 && x >= 2 && x <= d.nxp - 1;
+            if (ur) tur |= 1u << c;
+            if (wr) twr |= 1u << c;
+            if (ur || wr) cv[c] = CV[(unsigned)((x - 1) * d.ld + (z - 1))];
+        }
+    }
+
     for (int s = s_begin; s < s_end; ++s) {
         const ShotGeom g = geo[s];
         const float srcv = d.dt * src[(size_t)s * d.nt + (it - 1)];
-        const float *pin = In + (size_t)s * 3 * np;
-        // stage old state (zero outside the grid)
-        for (int i = tid; i < kNR; i += kThreads) {
-            int z = zo + i % kRZ, x = xo + i / kRZ;
-            float p = 0.f, u = 0.f, w = 0.f;
-            if (z >= 1 && z <= d.nzp && x >= 1 && x <= d.nxp) {
-                int k = (x - 1) * d.ld + (z - 1);
-                p = pin[k];
-                u = pin[np + k];
-                w = pin[2 * np + k];
+#pragma unroll
+        for (int j = 0; j < kLPT; ++j) {
+            int i = tid + j * kThreads;
+            if (i < kNR) {
+                sP[i] = lp[j];
+                sU[i] = lu[j];
+                sW[i] = lw[j];
             }
-            sP[i] = p;
-            sU[i] = u;
-            sW[i] = w;
         }
         __syncthreads();
+        if (s + 1 < s_end) stage_load(s + 1);   // next shot's loads overlap this shot
         // new p on rows z0-1..z0+TZ+1, cols x0-1..x0+TX+1
-        for (int i = tid; i < kNQ; i += kThreads) {
+#pragma unroll
+        for (int c = 0; c < kQPT; ++c) {
+            int i = tid + c * kThreads;
+            if (i >= kNQ) continue;
             int z = tb.z0 - 1 + i % kQZ, x = tb.x0 - 1 + i / kQZ;
             int l = L(z, x, zo, xo);
             float v = 0.f;
-            if (z >= 1 && z <= d.nzp && x >= 1 && x <= d.nxp) {
+            if (qgrid >> c & 1u) {
                 v = sP[l];
-                if (z >= fs + 1 && z <= d.nzp - 2 && x >= 3 && x <= d.nxp - 2) {
-                    float2 cp = CP[(x - 1) * d.ld + (z - 1)];
-                    v = p_update_f(v, cp.x, cp.y, sU[l], sU[l - kRZ], sU[l + kRZ], sU[l - 2 * kRZ],
+                if (qupd >> c & 1u) {
+                    v = p_update_f(v, cp[c].x, cp[c].y, sU[l], sU[l - kRZ], sU[l + kRZ], sU[l - 2 * kRZ],
                                    sW[l], sW[l - 1], sW[l + 1], sW[l - 2]);
                     if (z == g.isz && x == g.isx) v = v + srcv;
                 }
@@ -208,15 +293,9 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
         for (int c = 0; c < kCellsPerThread; ++c) {
             int i = tid + c * kThreads;
             int z = tb.z0 + i % kTZ, x = tb.x0 + i / kTZ;
-            if (z > tb.z1 || x > tb.x1) continue;
             int l = L(z, x, zo, xo);
-            bool ur = z >= fs && z <= d.nzp - 1 && x >= 2 && x <= d.nxp - 2;
-            bool wr = z >= fs && z <= d.nzp - 2 && x >= 2 && x <= d.nxp - 1;
-            if (ur || wr) {
-                float4 cv = CV[(x - 1) * d.ld + (z - 1)];
-                if (ur) sU[l] = v_update_f(sU[l], cv.x, cv.y, sQ[l], sQ[l + kRZ], sQ[l + 2 * kRZ], sQ[l - kRZ]);
-                if (wr) sW[l] = v_update_f(sW[l], cv.x, cv.z, sQ[l], sQ[l + 1], sQ[l + 2], sQ[l - 1]);
-            }
+            if (tur >> c & 1u) sU[l] = v_update_f(sU[l], cv[c].x, cv[c].y, sQ[l], sQ[l + kRZ], sQ[l + 2 * kRZ], sQ[l - kRZ]);
+            if (twr >> c & 1u) sW[l] = v_update_f(sW[l], cv[c].x, cv[c].z, sQ[l], sQ[l + 1], sQ[l + 2], sQ[l - 1]);
         }
         __syncthreads();
         // w mirror (fdcore.f90:116-120): rows fs-1 and fs share the tile that starts at fs-1
@@ -227,10 +306,11 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
         float *pout = Out + (size_t)s * 3 * np;
 #pragma unroll
         for (int c = 0; c < kCellsPerThread; ++c) {
+            if (!(tin >> c & 1u)) continue;
             int i = tid + c * kThreads;
             int z = tb.z0 + i % kTZ, x = tb.x0 + i / kTZ;
-            if (z > tb.z1 || x > tb.x1) continue;
-            int l = L(z, x, zo, xo), k = (x - 1) * d.ld + (z - 1);
+            int l = L(z, x, zo, xo);
+            const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
             pout[k] = sQ[l];
             pout[np + k] = sU[l];
             pout[2 * np + k] = sW[l];
@@ -261,6 +341,8 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
 // ---------------------------------------------------------------------------------
 // Backward step it (solver.f90:230-253): reconstruction (fdcore.f90:202-289), adjoint
 // step (fdcore.f90:293-375) and imaging (solver.f90:237-239); one shot per workgroup.
+// The reconstruction window and the adjoint window are staged together and their
+// velocity and pressure updates share barriers.
 // RIn/ROut: [ns][3][nx][nz]; BIn/BOut: [ns][3][nxp][ld].
 // acc: [ns][3][nx][nz] FP32 per-shot accumulators (kernel, forward illumination,
 // backward illumination), updated in the reference's order.
@@ -270,205 +352,267 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
     const float *__restrict__ RIn, float *__restrict__ ROut, const float *__restrict__ BIn,
     float *__restrict__ BOut, const int *__restrict__ inj_gstart, const InjGroup *__restrict__ inj_groups,
     const RecRef *__restrict__ inj_list, const float *__restrict__ sadj_t, float *__restrict__ acc) {
-    __shared__ float sP[kNR];
-    __shared__ float sU[kNR];
-    __shared__ float sW[kNR];
-    __shared__ float sQ[kNR];
+    using namespace bk;
+    __shared__ float rP[kNL];
+    __shared__ float rU[kNL];
+    __shared__ float rW[kNL];
+    __shared__ float bP[kNL];
+    __shared__ float bU[kNL];
+    __shared__ float bW[kNL];
+    __shared__ float bQ[kNL];
 
-    const int work = xcd_logical(blockIdx.x, gridDim.x);
-    const int s = work / tm.ntiles;
-    const int t = work % tm.ntiles;
+    int s, t;
+    work_item(xcd_logical(blockIdx.x, gridDim.x), (int)(gridDim.x / tm.ntiles), tm, s, t);
     const TileBox tb = tile_box(d, tm, t);
     const int zo = tb.z0 - kHalo, xo = tb.x0 - kHalo;
-    const int tid = threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int fs = d.fs, npml = d.npml, nx = d.nx, nz = d.nz;
-    const int np = d.nxp * d.ld;
-    const int ni = nx * nz;
-    // does this tile touch the interior window?
+    const int np = d.nxp * d.ld, ni = nx * nz;
     const bool has_interior = max(tb.z0, npml + 1) <= min(tb.z1, npml + nz) &&
                               max(tb.x0, npml + 1) <= min(tb.x1, npml + nx);
-    const int S = d.S;
     const int top0 = 3 * nx + 6 * nz;
-    const float *Bc = reinterpret_cast<const float *>(CV);   // B at 4*k
     const float *Kc = reinterpret_cast<const float *>(CP);   // K at 2*k, 1-kappa_p at 2*k+1
+    const float *Bc = reinterpret_cast<const float *>(CV);   // B at 4*k
     const ShotGeom g = geo[s];
+    const float *b = strips + ((size_t)s * d.nt + (it - 2)) * d.S;
+    // this thread's row
+    const int z = zo + lane, jz = z - npml;
+    const bool zin = z >= 1 && z <= d.nzp;
+    const bool rz = has_interior && jz >= 1 && jz <= nz;      // interior row
+    const bool zu = z >= 2 && z <= d.nzp - 1;                  // adjoint u rows (:311)
+    const bool zw = z >= 2 && z <= d.nzp - 2;                  // adjoint w rows (:320)
+    const bool zp = z >= 3 && z <= d.nzp - 2;                  // adjoint p rows (:337)
+    const bool lane_v = lane >= 1 && lane <= kZL - 3;          // velocity rows z0-2..z0+TZ
+    const bool lane_t = lane >= kHalo && z <= tb.z1;           // tile rows
+    // column sets: staged c in [0, kCX); velocities c in [1, cv_end]; tile c in [3, ct_end]
+    constexpr int kJS = (kCX + kWaves - 1) / kWaves;
+    constexpr int kJV = (kTX + 3 + kWaves - 1) / kWaves;
+    constexpr int kJT = (kTX + kWaves - 1) / kWaves;
+    const int cv_end = tb.x1 + 1 - xo;
+    const int ct_end = tb.x1 - xo;
 
-    float prn[kCellsPerThread];     // reconstructed p at it-1 on owned cells
-#pragma unroll
-    for (int c = 0; c < kCellsPerThread; ++c) prn[c] = 0.f;
-    // ---------------- reconstruction on the interior window -----------------
-    if (has_interior) {
-        const float *rin = RIn + (size_t)s * 3 * ni;
-        const float *b = strips + ((size_t)s * d.nt + (it - 2)) * S;
-        for (int i = tid; i < kNR; i += kThreads) {
-            int jz = zo + i % kRZ - npml, jx = xo + i / kRZ - npml;
-            float p = 0.f, u = 0.f, w = 0.f;
-            if (jz >= 1 && jz <= nz && jx >= 1 && jx <= nx) {
-                int k = (jx - 1) * nz + (jz - 1);
-                p = rin[k];
-                u = rin[ni + k];
-                w = rin[2 * ni + k];
-            }
-            sP[i] = p;
-            sU[i] = u;
-            sW[i] = w;
-        }
-        __syncthreads();
-        // u', w' on rows z0-2..z0+TZ, cols x0-2..x0+TX (fdcore.f90:221-247)
-        for (int i = tid; i < kNQ; i += kThreads) {
-            int z = tb.z0 - 2 + i % kQZ, x = tb.x0 - 2 + i / kQZ;
-            int jz = z - npml, jx = x - npml;
-            if (jz < 1 || jz > nz || jx < 1 || jx > nx) continue;
-            int l = L(z, x, zo, xo);
-            if (jx == 1 || jx == nx - 1) {
-                int jj = 3 * nx + 6 * (jz - 1);
-                sU[l] = jx == 1 ? b[jj + 2] : b[jj + 5];
-            } else if (jx >= 2 && jx <= nx - 2 && jz >= 2 && jz <= nz - 1) {
-                float alpha = Bc[4 * ((x - 1) * d.ld + (z - 1))];
-                sU[l] = v_undo(sU[l], alpha, sP[l], sP[l + kRZ], sP[l + 2 * kRZ], sP[l - kRZ]);
-            }
-            if (jz == nz - 1) {
-                sW[l] = b[3 * (jx - 1) + 2];
-            } else if (jx >= 2 && jx <= nx - 1 && jz >= 2 && jz <= nz - 2) {
-                float alpha = Bc[4 * ((x - 1) * d.ld + (z - 1))];
-                sW[l] = v_undo(sW[l], alpha, sP[l], sP[l + 1], sP[l + 2], sP[l - 1]);
-            }
-        }
-        __syncthreads();
-        // p' on owned tile cells (fdcore.f90:250-287)
-        float *rout = ROut + (size_t)s * 3 * ni;
-#pragma unroll
-        for (int c = 0; c < kCellsPerThread; ++c) {
-            int i = tid + c * kThreads;
-            int z = tb.z0 + i % kTZ, x = tb.x0 + i / kTZ;
-            int jz = z - npml, jx = x - npml;
-            if (z > tb.z1 || x > tb.x1 || jz < 1 || jz > nz || jx < 1 || jx > nx) continue;
-            int l = L(z, x, zo, xo);
-            float v;
-            if (jz <= 3) {
-                v = b[top0 + 6 * (jx - 1) + (jz - 1)];
-            } else if (jx <= 2 || jx >= nx - 1) {
-                int jj = 3 * nx + 6 * (jz - 1);
-                v = jx == 1 ? b[jj] : jx == 2 ? b[jj + 1] : jx == nx ? b[jj + 3] : b[jj + 4];
-            } else if (jz >= nz - 1) {
-                int jj = 3 * (jx - 1);
-                v = jz == nz ? b[jj] : b[jj + 1];
-            } else {
-                v = sP[l];
-                if (jz == g.isz_i && jx == g.isx_i && g.isx_i > 2 && g.isx_i < nx - 1 && g.isz_i > 3 &&
-                    g.isz_i < nz - 1) {
-                    // the reference reads s(it+1), one past the end at it = nt (taken as 0)
-                    float sv = it + 1 <= d.nt ? src[(size_t)s * d.nt + it] : 0.0f;
-                    v = v - d.dt * sv;
-                }
-                float alpha = Kc[2 * ((x - 1) * d.ld + (z - 1))];
-                v = p_undo(v, alpha, sU[l], sU[l - kRZ], sU[l + kRZ], sU[l - 2 * kRZ],
-                           sW[l], sW[l - 1], sW[l + 1], sW[l - 2]);
-            }
-            int k = (jx - 1) * nz + (jz - 1);
-            rout[k] = v;
-            rout[ni + k] = sU[l];
-            rout[2 * ni + k] = sW[l];
-            prn[c] = v;
-        }
-        __syncthreads();
-    }
-    // ---------------- adjoint step on the padded grid ------------------------
-    const float *bin = BIn + (size_t)s * 3 * np;
-    for (int i = tid; i < kNR; i += kThreads) {
-        int z = zo + i % kRZ, x = xo + i / kRZ;
-        float p = 0.f, u = 0.f, w = 0.f;
-        if (z >= 1 && z <= d.nzp && x >= 1 && x <= d.nxp) {
-            int k = (x - 1) * d.ld + (z - 1);
-            p = bin[k];
-            u = bin[np + k];
-            w = bin[2 * np + k];
-        }
-        sP[i] = p;
-        sU[i] = u;
-        sW[i] = w;
-    }
-    __syncthreads();
-    // u', w' with alpha = den c^2 dt/dx on the region (fdcore.f90:310-326)
-    for (int i = tid; i < kNQ; i += kThreads) {
-        int z = tb.z0 - 2 + i % kQZ, x = tb.x0 - 2 + i / kQZ;
-        bool ur = z >= 2 && z <= d.nzp - 1 && x >= 2 && x <= d.nxp - 2;
-        bool wr = z >= 2 && z <= d.nzp - 2 && x >= 2 && x <= d.nxp - 1;
-        if (!(ur || wr)) continue;
-        int l = L(z, x, zo, xo);
-        int k = (x - 1) * d.ld + (z - 1);
-        float alpha = Kc[2 * k];
-        float4 cv = CV[k];
-        if (ur) sU[l] = v_update_f(sU[l], alpha, cv.y, sP[l], sP[l + kRZ], sP[l + 2 * kRZ], sP[l - kRZ]);
-        if (wr) sW[l] = v_update_f(sW[l], alpha, cv.z, sP[l], sP[l + 1], sP[l + 2], sP[l - 1]);
-    }
-    __syncthreads();
-    // w mirror (fdcore.f90:329-333), columns 2..nxp-1 of the region
-    if (d.free_surface == 1 && fs - 1 >= tb.z0 - 2 && fs <= tb.z0 + kTZ) {
-        for (int x = tb.x0 - 2 + tid; x <= tb.x0 + kTX; x += kThreads)
-            if (x >= 2 && x <= d.nxp - 1) sW[L(fs - 1, x, zo, xo)] = sW[L(fs, x, zo, xo)];
-    }
-    __syncthreads();
-    // p' with alpha = dt/(den dx) on the tile into sQ (fdcore.f90:336-345)
-#pragma unroll
-    for (int c = 0; c < kCellsPerThread; ++c) {
-        int i = tid + c * kThreads;
-        int z = tb.z0 + i % kTZ, x = tb.x0 + i / kTZ;
-        if (z > tb.z1 || x > tb.x1) continue;
-        int l = L(z, x, zo, xo);
-        float v = sP[l];
-        if (z >= 3 && z <= d.nzp - 2 && x >= 3 && x <= d.nxp - 2) {
-            int k = (x - 1) * d.ld + (z - 1);
-            float alpha = Bc[4 * k];
-            float fac = Kc[2 * k + 1];
-            v = p_update_f(v, alpha, fac, sU[l], sU[l - kRZ], sU[l + kRZ], sU[l - 2 * kRZ],
-                           sW[l], sW[l - 1], sW[l + 1], sW[l - 2]);
-        }
-        sQ[l] = v;
-    }
-    __syncthreads();
-    // adjoint-source injection (fdcore.f90:348-367): one thread per cell, receivers
-    // sharing a cell added in receiver order
+    // ---- entry loads: staged windows, coefficients, strip inserts, accumulators ----
+    float rp[kJS], ru[kJS], rw[kJS], bp[kJS], bu[kJS], bw[kJS];
     {
-        int g0 = inj_gstart[(size_t)s * (tm.ntiles + 1) + t];
-        int g1 = inj_gstart[(size_t)s * (tm.ntiles + 1) + t + 1];
-        const RecRef *el = inj_list + (size_t)s * d.ngmax;
-        const float *sa = sadj_t + ((size_t)s * d.nt + (it - 1)) * d.ngmax;
-        for (int q = g0 + tid; q < g1; q += kThreads) {
-            InjGroup gr = inj_groups[(size_t)s * d.ngmax + q];
-            RecRef r0 = el[gr.e0];
-            int l = L(r0.iz, r0.ix, zo, xo);
-            float v = sQ[l];
-            for (int e = gr.e0; e < gr.e1; ++e) v = v + d.dt * sa[el[e].idx];
-            sQ[l] = v;
+        const float *rin = RIn + (size_t)s * 3 * ni;
+        const float *bin = BIn + (size_t)s * 3 * np;
+#pragma unroll
+        for (int j = 0; j < kJS; ++j) {
+            int c = wave + kWaves * j;
+            int x = xo + c, jx = x - npml;
+            rp[j] = ru[j] = rw[j] = bp[j] = bu[j] = bw[j] = 0.f;
+            if (c < kCX && rz && jx >= 1 && jx <= nx) {
+                const unsigned k = (unsigned)((jx - 1) * nz + (jz - 1));
+                rp[j] = rin[k];
+                ru[j] = rin[ni + k];
+                rw[j] = rin[2 * ni + k];
+            }
+            if (c < kCX && zin && x >= 1 && x <= d.nxp) {
+                const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
+                bp[j] = bin[k];
+                bu[j] = bin[np + k];
+                bw[j] = bin[2 * np + k];
+            }
         }
     }
-    __syncthreads();
-    // unconditional p mirror (fdcore.f90:370-373): rows fs-1 and fs+1 share a tile
-    if (fs - 1 >= tb.z0 && fs + 1 <= tb.z1) {
-        for (int x = tb.x0 + tid; x <= tb.x1; x += kThreads) sQ[L(fs - 1, x, zo, xo)] = -sQ[L(fs + 1, x, zo, xo)];
+    // velocity cells: vq = {B (reconstruction alpha), 1-kappa_u, 1-kappa_w}, vk = K
+    // (adjoint alpha), vsu / vsw = strip values inserted into the reconstructed u / w
+    float4 vq[kJV];
+    float vk[kJV], vsu[kJV], vsw[kJV];
+#pragma unroll
+    for (int j = 0; j < kJV; ++j) {
+        int c = 1 + wave + kWaves * j;
+        int x = xo + c, jx = x - npml;
+        vq[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+        vk[j] = vsu[j] = vsw[j] = 0.f;
+        if (c > cv_end || !lane_v || !zin || x < 1 || x > d.nxp) continue;
+        const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
+        bool au = zu && x >= 2 && x <= d.nxp - 2;
+        bool aw = zw && x >= 2 && x <= d.nxp - 1;
+        bool rc = rz && jx >= 1 && jx <= nx;
+        if (au || aw || rc) vq[j] = CV[k];
+        if (au || aw) vk[j] = Kc[2 * k];
+        if (rc) {
+            if (jx == 1 || jx == nx - 1) vsu[j] = b[3 * nx + 6 * (jz - 1) + (jx == 1 ? 2 : 5)];
+            if (jz == nz - 1) vsw[j] = b[3 * (jx - 1) + 2];
+        }
     }
-    __syncthreads();
-    // write back; imaging dg += p_rec (p0 - p), dfw += p_rec^2, dbk += p^2
-    float *bout = BOut + (size_t)s * 3 * np;
+    // tile cells: tk = {K (reconstruction alpha), 1-kappa_p}, tbv = B (adjoint alpha),
+    // tsp = strip value inserted into the reconstructed p, ta* = this shot's accumulators
+    float2 tk[kJT];
+    float tbv[kJT], tsp[kJT], ta0[kJT], ta1[kJT], ta2[kJT];
     float *a = acc + (size_t)s * 3 * ni;
 #pragma unroll
-    for (int c = 0; c < kCellsPerThread; ++c) {
-        int i = tid + c * kThreads;
-        int z = tb.z0 + i % kTZ, x = tb.x0 + i / kTZ;
-        if (z > tb.z1 || x > tb.x1) continue;
-        int l = L(z, x, zo, xo), k = (x - 1) * d.ld + (z - 1);
-        float pb = sQ[l];
+    for (int j = 0; j < kJT; ++j) {
+        int c = kHalo + wave + kWaves * j;
+        int x = xo + c, jx = x - npml;
+        tk[j] = make_float2(0.f, 0.f);
+        tbv[j] = tsp[j] = ta0[j] = ta1[j] = ta2[j] = 0.f;
+        if (c > ct_end || !lane_t) continue;
+        const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
+        tk[j] = CP[k];
+        if (zp && x >= 3 && x <= d.nxp - 2) tbv[j] = Bc[4 * k];
+        if (rz && jx >= 1 && jx <= nx) {
+            const unsigned ki = (unsigned)((jx - 1) * nz + (jz - 1));
+            ta0[j] = a[ki];
+            ta1[j] = a[ni + ki];
+            ta2[j] = a[2 * ni + ki];
+            if (jz <= 3) tsp[j] = b[top0 + 6 * (jx - 1) + (jz - 1)];
+            else if (jx <= 2 || jx >= nx - 1)
+                tsp[j] = b[3 * nx + 6 * (jz - 1) + (jx == 1 ? 0 : jx == 2 ? 1 : jx == nx ? 3 : 4)];
+            else if (jz >= nz - 1) tsp[j] = b[3 * (jx - 1) + (jz == nz ? 0 : 1)];
+        }
+    }
+    // first adjoint-source sample of this thread's injection cell
+    const int g0 = inj_gstart[(size_t)s * (tm.ntiles + 1) + t];
+    const int g1 = inj_gstart[(size_t)s * (tm.ntiles + 1) + t + 1];
+    const RecRef *el = inj_list + (size_t)s * d.ngmax;
+    const float *sa = sadj_t + ((size_t)s * d.nt + (it - 1)) * d.ngmax;
+    InjGroup ig0{0, 0};
+    RecRef ir0{0, 0, 0, 0};
+    float iv0 = 0.f;
+    if (g0 + (int)threadIdx.x < g1) {
+        ig0 = inj_groups[(size_t)s * d.ngmax + g0 + threadIdx.x];
+        ir0 = el[ig0.e0];
+        iv0 = sa[ir0.idx];
+    }
+    // the reference's s(it+1) for the reconstruction source subtraction (0 past the end)
+    const float ssub = it + 1 <= d.nt ? src[(size_t)s * d.nt + it] : 0.0f;
+
+    // ---- stage both windows ------------------------------------------------------
+#pragma unroll
+    for (int j = 0; j < kJS; ++j) {
+        int c = wave + kWaves * j;
+        if (c < kCX) {
+            int l = LW(c, lane);
+            rP[l] = rp[j];
+            rU[l] = ru[j];
+            rW[l] = rw[j];
+            bP[l] = bp[j];
+            bU[l] = bu[j];
+            bW[l] = bw[j];
+        }
+    }
+    __syncthreads();
+    // ---- velocities: reconstruction u', w' (:221-247), adjoint u', w' (:310-326) ----
+#pragma unroll
+    for (int j = 0; j < kJV; ++j) {
+        int c = 1 + wave + kWaves * j;
+        if (c > cv_end || !lane_v || !zin) continue;
+        int x = xo + c, jx = x - npml;
+        if (x < 1 || x > d.nxp) continue;
+        int l = LW(c, lane);
+        if (rz && jx >= 1 && jx <= nx) {
+            if (jx == 1 || jx == nx - 1) rU[l] = vsu[j];
+            else if (jx <= nx - 2 && jz >= 2 && jz <= nz - 1)
+                rU[l] = v_undo(rU[l], vq[j].x, rP[l], rP[l + kZL], rP[l + 2 * kZL], rP[l - kZL]);
+            if (jz == nz - 1) rW[l] = vsw[j];
+            else if (jx >= 2 && jx <= nx - 1 && jz >= 2 && jz <= nz - 2)
+                rW[l] = v_undo(rW[l], vq[j].x, rP[l], rP[l + 1], rP[l + 2], rP[l - 1]);
+        }
+        if (zu && x >= 2 && x <= d.nxp - 2)
+            bU[l] = v_update_f(bU[l], vk[j], vq[j].y, bP[l], bP[l + kZL], bP[l + 2 * kZL], bP[l - kZL]);
+        if (zw && x >= 2 && x <= d.nxp - 1)
+            bW[l] = v_update_f(bW[l], vk[j], vq[j].z, bP[l], bP[l + 1], bP[l + 2], bP[l - 1]);
+    }
+    __syncthreads();
+    // adjoint w mirror (:329-333), columns 2..nxp-1
+    if (d.free_surface == 1) {
+        const int lm = fs - 1 - zo;
+        if (lane == lm && lm >= 1 && lm + 1 <= kZL - 3) {
+#pragma unroll
+            for (int j = 0; j < kJV; ++j) {
+                int c = 1 + wave + kWaves * j;
+                int x = xo + c;
+                if (c <= cv_end && x >= 2 && x <= d.nxp - 1) bW[LW(c, lm)] = bW[LW(c, lm + 1)];
+            }
+        }
+    }
+    __syncthreads();
+    // ---- pressures: reconstruction p' (:250-287), adjoint p' (:336-345) ------------
+    float prn[kJT];
+    float *rout = ROut + (size_t)s * 3 * ni;
+#pragma unroll
+    for (int j = 0; j < kJT; ++j) {
+        prn[j] = 0.f;
+        int c = kHalo + wave + kWaves * j;
+        if (c > ct_end || !lane_t) continue;
+        int x = xo + c, jx = x - npml;
+        int l = LW(c, lane);
+        if (rz && jx >= 1 && jx <= nx) {
+            float v;
+            if (jz <= 3 || jx <= 2 || jx >= nx - 1 || jz >= nz - 1) {
+                v = tsp[j];
+            } else {
+                v = rP[l];
+                if (jz == g.isz_i && jx == g.isx_i && g.isx_i > 2 && g.isx_i < nx - 1 && g.isz_i > 3 &&
+                    g.isz_i < nz - 1)
+                    v = v - d.dt * ssub;
+                v = p_undo(v, tk[j].x, rU[l], rU[l - kZL], rU[l + kZL], rU[l - 2 * kZL],
+                           rW[l], rW[l - 1], rW[l + 1], rW[l - 2]);
+            }
+            const unsigned k = (unsigned)((jx - 1) * nz + (jz - 1));
+            rout[k] = v;
+            rout[ni + k] = rU[l];
+            rout[2 * ni + k] = rW[l];
+            prn[j] = v;
+        }
+        float v = bP[l];
+        if (zp && x >= 3 && x <= d.nxp - 2)
+            v = p_update_f(v, tbv[j], tk[j].y, bU[l], bU[l - kZL], bU[l + kZL], bU[l - 2 * kZL],
+                           bW[l], bW[l - 1], bW[l + 1], bW[l - 2]);
+        bQ[l] = v;
+    }
+    __syncthreads();
+    // adjoint-source injection (:348-367): one thread per cell, receivers sharing a cell
+    // added in receiver order
+    if (g0 + (int)threadIdx.x < g1) {
+        int l = LW(ir0.ix - xo, ir0.iz - zo);
+        float v = bQ[l] + d.dt * iv0;
+        for (int e = ig0.e0 + 1; e < ig0.e1; ++e) v = v + d.dt * sa[el[e].idx];
+        bQ[l] = v;
+    }
+    for (int q = g0 + (int)threadIdx.x + kThreads; q < g1; q += kThreads) {
+        InjGroup gr = inj_groups[(size_t)s * d.ngmax + q];
+        RecRef r0 = el[gr.e0];
+        int l = LW(r0.ix - xo, r0.iz - zo);
+        float v = bQ[l];
+        for (int e = gr.e0; e < gr.e1; ++e) v = v + d.dt * sa[el[e].idx];
+        bQ[l] = v;
+    }
+    __syncthreads();
+    // unconditional p mirror (:370-373): rows fs-1 and fs+1 are rows of one tile
+    {
+        const int lm = fs - 1 - zo;
+        if (lane == lm && lm >= kHalo && fs + 1 <= tb.z1) {
+#pragma unroll
+            for (int j = 0; j < kJT; ++j) {
+                int c = kHalo + wave + kWaves * j;
+                if (c <= ct_end) bQ[LW(c, lm)] = -bQ[LW(c, lm + 2)];
+            }
+        }
+    }
+    __syncthreads();
+    // ---- write back; imaging dg += p_rec (p0 - p), dfw += p_rec^2, dbk += p^2 -------
+    float *bout = BOut + (size_t)s * 3 * np;
+#pragma unroll
+    for (int j = 0; j < kJT; ++j) {
+        int c = kHalo + wave + kWaves * j;
+        if (c > ct_end || !lane_t) continue;
+        int x = xo + c, jx = x - npml;
+        int l = LW(c, lane);
+            const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
+        float pb = bQ[l];
         bout[k] = pb;
-        bout[np + k] = sU[l];
-        bout[2 * np + k] = sW[l];
-        int jz = z - npml, jx = x - npml;
-        if (jz >= 1 && jz <= nz && jx >= 1 && jx <= nx) {
-            int ki = (jx - 1) * nz + (jz - 1);
-            float pr = prn[c];
-            a[ki] = a[ki] + pr * (sP[l] - pb);
-            a[ni + ki] = a[ni + ki] + pr * pr;
-            a[2 * ni + ki] = a[2 * ni + ki] + pb * pb;
+        bout[np + k] = bU[l];
+        bout[2 * np + k] = bW[l];
+        if (rz && jx >= 1 && jx <= nx) {
+            const unsigned ki = (unsigned)((jx - 1) * nz + (jz - 1));
+            float pr = prn[j];
+            a[ki] = ta0[j] + pr * (bP[l] - pb);
+            a[ni + ki] = ta1[j] + pr * pr;
+            a[2 * ni + ki] = ta2[j] + pb * pb;
         }
     }
 }

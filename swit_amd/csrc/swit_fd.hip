@@ -396,18 +396,44 @@ extern "C" int swit_adjoint(swit_plan *pl, const float *src, const float *adjsrc
 
 // ---- batched path (host side) --------------------------------------------------------
 
-static TileMap make_tilemap(const Dims &d, int ztop) {
+static TileMap make_tilemap(const Dims &d, int ztop, int th) {
     TileMap tm;
     tm.ztop = ztop;
-    tm.nzt = cdiv(d.nzp - ztop + 1, kTZ);
+    tm.th = th;
+    tm.nzt = cdiv(d.nzp - ztop + 1, th);
     tm.nxt = cdiv(d.nxp, kTX);
     tm.ntiles = tm.nzt * tm.nxt;
     return tm;
 }
 
 static int tile_of(const TileMap &tm, int iz, int ix) {
-    int zt = (iz - tm.ztop) / kTZ, xt = (ix - 1) / kTX;
+    int zt = (iz - tm.ztop) / tm.th, xt = (ix - 1) / kTX;
     return xt * tm.nzt + zt;
+}
+
+// Backward tiling: rows 2..nzp-1 are the only rows a backward step changes (adjoint
+// u/w rows 2..nzp-1, p rows 3..nzp-2; the interior window lies inside), and rows
+// fs-2..fs+1 must share a tile (the adjoint p stencil there reads the mirrored w(fs-1);
+// the p mirror reads row fs+1 after injection).  Pick the offset with the fewest tiles.
+static TileMap backward_tilemap(const Dims &d) {
+    const int th = bk::kTZ, fs = d.fs;
+    TileMap best{};
+    best.ntiles = 1 << 30;
+    for (int shift = 0; shift < th; ++shift) {
+        int ztop = 2 - shift;
+        bool ok = true;
+        for (int b = ztop + th; b <= d.nzp; b += th)       // tile starts after the first
+            if (b >= fs - 1 && b <= fs + 1) ok = false;
+        if (!ok) continue;
+        TileMap tm;
+        tm.ztop = ztop;
+        tm.th = th;
+        tm.nzt = cdiv(d.nzp - 1 - ztop + 1, th);           // cover rows up to nzp-1
+        tm.nxt = cdiv(d.nxp, kTX);
+        tm.ntiles = tm.nzt * tm.nxt;
+        if (tm.ntiles < best.ntiles) best = tm;
+    }
+    return best;
 }
 
 static int fill(float *p, size_t n, hipStream_t st) {
@@ -430,11 +456,8 @@ static int batched_init(swit_plan *pl) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
     const int ns = pl->ns, ngmax = d.ngmax, fs = d.fs;
-    bs.fwd = make_tilemap(d, fs - 1);
-    // Backward tiles start at row fs-3 so rows fs-2..fs+1 share one tile: the adjoint
-    // p stencil at rows fs-2..fs+1 reads the mirrored w(fs-1), and the p mirror reads
-    // row fs+1 after injection (fdcore.f90:329-333, :370-373).
-    bs.bwd = make_tilemap(d, (fs - 3) - kTZ * cdiv(fs - 4, kTZ));
+    bs.fwd = make_tilemap(d, fs - 1, fk::kTZ);
+    bs.bwd = backward_tilemap(d);
     // receivers: recording cells (forward tiling) and injection cells (backward tiling)
     std::vector<int> rs((size_t)ns * (bs.fwd.ntiles + 1)), is((size_t)ns * (bs.bwd.ntiles + 1));
     std::vector<RecRef> rl((size_t)ns * ngmax), il((size_t)ns * ngmax);
@@ -536,11 +559,8 @@ static int batched_init(swit_plan *pl) {
         HIP_TRY(hipMemset(bs.F[k], 0, 3 * np * ns * sizeof(float)));
         HIP_TRY(hipMemset(bs.Bk[k], 0, 3 * np * ns * sizeof(float)));
     }
-    int Gf = 1;
-    const char *ef = getenv("SWIT_FWD_GROUP");
-    if (ef && atoi(ef) > 0) Gf = std::min(atoi(ef), ns);
-    bs.Gf = Gf;
-    bs.ngroups_f = cdiv(ns, Gf);
+    bs.Gf = std::min(4, ns);
+    bs.ngroups_f = cdiv(ns, bs.Gf);
     const char *eg = getenv("SWIT_NO_GRAPH");
     bs.use_graph = !(eg && atoi(eg) > 0);
     HIP_TRY(hipStreamCreateWithFlags(&bs.cap, hipStreamNonBlocking));

@@ -186,10 +186,11 @@ def small(seed=0, nx=64, nz=40, npml=10, nt=400, ns=3, dx=10.0, dt=0.001):
     init = gaussian_filter(vp.astype(np.float64), 4, mode="nearest").astype(np.float32)
     sx = np.linspace(3, nx - 4, ns)
     sz = np.array([0, 2, 7, 1, 3, 12][:ns] + [2] * max(0, ns - 6))[:ns]
+    sz = np.minimum(sz, nz - 4)
     src = np.stack([sx * dx, sz * dx], 1).astype(np.float32)
     rec = np.zeros((nx + 2, 2), np.float32)
     rec[:nx, 0] = np.arange(nx) * dx
-    rec[nx] = [(nx // 3) * dx + 3.0, 15 * dx + 4.0]
+    rec[nx] = [(nx // 3) * dx + 3.0, min(15, nz - 3) * dx + 4.0]
     rec[nx + 1] = [(2 * nx // 3) * dx, 1 * dx]
     return Workload("small", nx, nz, dx, dt, nt, npml, 12.0, vp, init, src,
                     np.broadcast_to(rec, (ns,) + rec.shape).copy())

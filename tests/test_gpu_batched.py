@@ -93,3 +93,23 @@ def test_profile_counts_every_step():
     p.close()
     assert fn == w.nt - 1 and bn == w.nt - 1
     assert fm > 0 and bm > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nx,nz,npml,nt,ns", [
+    (40, 30, 59, 160, 2),     # free-surface rows straddle the default backward tile boundary
+    (70, 150, 20, 220, 3),    # several tile rows in both step kernels
+    (33, 9, 3, 150, 2),       # thinnest absorbing layer and interior the plan accepts
+    (100, 40, 12, 200, 4),    # several tile columns, uneven last column
+])
+def test_batched_tilings(nx, nz, npml, nt, ns):
+    w, src, adj, f, a = _case(seed=nx + nz, nx=nx, nz=nz, npml=npml, nt=nt, ns=ns)
+    for graph in (1, 0):
+        p = _plan(w)
+        p.set_option("graph", graph)
+        seis, (g, fw, bk) = _run(p, src, adj)
+        p.close()
+        np.testing.assert_array_equal(seis, f["seis"])
+        assert rel_l2(g, a["grad"]) <= 1e-12, (graph, rel_l2(g, a["grad"]))
+        assert rel_l2(fw, a["illum_forw"]) <= 1e-12
+        assert rel_l2(bk, a["illum_back"]) <= 1e-12
