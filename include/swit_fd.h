@@ -85,7 +85,9 @@ int swit_plan_set_model(swit_plan *plan, const float *vp, const float *rho,
  *         sample 0 is zero as in the reference.
  *   save_strips: 1 keeps boundary strips and the final state on the device for a
  *         following swit_adjoint (the reference's STORE_SNAP=1 files,
- *         fdcore.f90:130-176); 0 invalidates them (the reference's cleandata). */
+ *         fdcore.f90:130-176); 0 invalidates them (the reference's cleandata of
+ *         data/syn/); 2 saves nothing and leaves earlier saved strips valid (an
+ *         observed-data run, whose files the reference writes under data/obs/). */
 int swit_forward(swit_plan *plan, const float *src, float *seis, int save_strips,
                  hipStream_t stream);
 

@@ -357,16 +357,18 @@ extern "C" int swit_forward(swit_plan *pl, const float *src, float *seis, int sa
     if (!pl || !src || !seis) return set_err(SWIT_EINVAL, "plan, src and seis are required");
     if (!pl->model_set) return set_err(SWIT_ESTATE, "swit_plan_set_model must be called first");
     const Dims &d = pl->d;
-    pl->strips_valid = false;
+    if (save_strips < 0 || save_strips > 2) return set_err(SWIT_EINVAL, "save_strips must be 0, 1 or 2");
+    const int save = save_strips == 1;
+    if (save_strips != 2) pl->strips_valid = false;
     if (pl->flags & SWIT_PATH_BATCHED)
-        TRY(batched_forward(pl, src, save_strips, stream));
+        TRY(batched_forward(pl, src, save, stream));
     else
-        TRY(plain_forward(pl, src, save_strips, stream));
+        TRY(plain_forward(pl, src, save, stream));
     // [ns][nt][ngmax] -> [ns][ngmax][nt]
     hipLaunchKernelGGL(k_transpose, dim3(cdiv(d.ngmax, 32), cdiv(d.nt, 32), pl->ns), dim3(32, 8), 0,
                        stream, d.nt, d.ngmax, pl->seis_t, seis);
     LAUNCH_CHECK();
-    pl->strips_valid = save_strips != 0;
+    if (save_strips != 2) pl->strips_valid = save_strips == 1;
     return SWIT_OK;
 }
 

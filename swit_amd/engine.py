@@ -115,13 +115,16 @@ class Plan:
                                                     self._stream()))
 
     def forward(self, src, save_strips=True, out=None):
-        """src: (ns, nt) integrated source traces -> seismograms (ns, ngmax, nt)."""
+        """src: (ns, nt) integrated source traces -> seismograms (ns, ngmax, nt).
+        save_strips: 1/True save strips for adjoint(), 0/False discard saved strips,
+        2 save nothing but keep earlier saved strips."""
         torch = self._torch
+        save_strips = int(save_strips)
         src = self._dev(src, torch.float32, (self.ns, self.nt))
         if out is None:
             out = torch.empty((self.ns, self.ngmax, self.nt), dtype=torch.float32, device=self.device)
         _native.check(self._lib.swit_forward(self._h, ctypes.c_void_p(src.data_ptr()),
-                                             ctypes.c_void_p(out.data_ptr()), int(bool(save_strips)),
+                                             ctypes.c_void_p(out.data_ptr()), save_strips,
                                              self._stream()))
         self._src = src
         return out

@@ -1,0 +1,94 @@
+"""The drop-in solver (swit_amd.solver, toolbox/solver.py's surface) on the GPU against
+the oracle: SU seismograms and the raw kernel / illumination files bit-exact, the
+preconditioned gradient identical, for the reference call sequence
+(toolbox/inversion.py:48,54,57,60)."""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import integrated_sources
+from oracle import oracle as O
+from swit_amd import workloads
+
+
+def _simu(home, w, vp):
+    from swit_amd import base
+    model = base.model(w.nx, w.nz, w.dx, w.dt, w.nt, True, w.npml, vp.astype(np.float64),
+                       workloads.gardner(vp).astype(np.float64))
+    wav = np.broadcast_to(workloads.ricker(w.nt, w.dt, w.f0), (w.ns, w.nt)).copy()
+    source = base.source(w.f0, w.ns, w.src_xz.astype(np.float64), wav)
+    receiver = base.receiver(w.rec_xz.shape[1], w.rec_xz.astype(np.float64))
+    return base.simulate(model, source, receiver, base.system(home, 4))
+
+
+def _optim():
+    from swit_amd import base
+    return base.optimize('Waveform', 'LBFGS', 1, 0.05, 3500.0, 1400.0, 'Marine', 3, 2, 'None', 1.0, 10.0,
+                         False, 0.0, 'None', False, False, 0.0, 1e9)
+
+
+@pytest.mark.gpu
+def test_dropin_sequence_matches_oracle(tmp_path):
+    from swit_amd import misfit, postprocess, preprocess, solver, su, tools
+    w = workloads.small(seed=21, nx=56, nz=36, npml=10, nt=300, ns=3)
+    w.rec_xz = w.rec_xz[:, :w.nx]                     # surface receivers, all inside x range
+    home = str(tmp_path) + "/"
+    simu_true = _simu(home, w, w.vp_true)
+    simu = _simu(home, w, w.vp_init)
+    optim = _optim()
+    solver.forward(simu_true, 'obs', savesnap=0)
+    solver.forward(simu, 'syn', savesnap=1)
+    assert os.getcwd() == os.path.realpath(home.rstrip("/"))
+    preprocess.process_workflow(simu, optim, 'obs')
+    preprocess.process_workflow(simu, optim, 'syn')
+    J = misfit.misfit(simu, 'Waveform')
+    g = solver.adjoint(simu, optim)
+
+    src = integrated_sources(w)
+    inp = solver.solver_inputs(simu)
+    mt = O.Model(w.vp_true, workloads.gardner(w.vp_true), w.dx, w.npml, "f32")
+    m = O.Model(w.vp_init, workloads.gardner(w.vp_init), w.dx, w.npml, "f32")
+    obs = O.forward(mt, w.nt, inp["dt"], inp["src_xz"], inp["rec_xz"], src, save_strips=False)["seis"]
+    f = O.forward(m, w.nt, inp["dt"], inp["src_xz"], inp["rec_xz"], src)
+    for i in range(w.ns):
+        d_obs, _ = su.read(home + 'data/obs/src%d_sg.su' % (i + 1))
+        d_syn, _ = su.read(home + 'data/syn/src%d_sg.su' % (i + 1))
+        np.testing.assert_array_equal(d_obs, obs[i])
+        np.testing.assert_array_equal(d_syn, f["seis"][i])
+    adj = (f["seis"].astype(np.float64) - obs.astype(np.float64))
+    a = O.adjoint(m, w.nt, inp["dt"], inp["src_xz"], inp["rec_xz"], src, adj.astype(np.float32), f)
+    kern = tools.loadbinfloat32(home + 'data/syn/src0_kernel_vp.bin')
+    np.testing.assert_array_equal(kern, a["grad"].astype(np.float32).ravel())
+    np.testing.assert_array_equal(tools.loadbinfloat32(home + 'data/syn/src0_illum_forw.bin'),
+                                  a["illum_forw"].astype(np.float32).ravel())
+    ref = postprocess.grad_precond(simu, optim, a["grad"].astype(np.float32).ravel(),
+                                   a["illum_forw"].astype(np.float32).ravel(),
+                                   a["illum_back"].astype(np.float32).ravel())
+    np.testing.assert_array_equal(g, ref)
+    r = f["seis"].astype(np.float64) - obs
+    assert J == pytest.approx(sum(misfit.misfit_waveform(obs[i], f["seis"][i], inp["dt"]) for i in range(w.ns)))
+    assert J > 0
+    solver.release()
+
+
+@pytest.mark.gpu
+def test_obs_forward_keeps_saved_state_and_syn_without_snapshots_clears_it(tmp_path):
+    from swit_amd import preprocess, solver
+    w = workloads.small(seed=22, nx=40, nz=30, npml=8, nt=200, ns=2)
+    w.rec_xz = w.rec_xz[:, :w.nx]
+    home = str(tmp_path) + "/"
+    simu = _simu(home, w, w.vp_init)
+    simu_true = _simu(home, w, w.vp_true)
+    optim = _optim()
+    solver.forward(simu, 'syn', savesnap=1)
+    solver.forward(simu_true, 'obs', savesnap=0)       # must not discard the saved state
+    preprocess.process_workflow(simu, optim, 'obs')
+    preprocess.process_workflow(simu, optim, 'syn')
+    g = solver.adjoint(simu, optim)
+    assert np.all(np.isfinite(g)) and np.abs(g).max() > 0
+    solver.forward(simu, 'syn', savesnap=0)            # line-search forward: state gone
+    preprocess.process_workflow(simu, optim, 'syn')
+    with pytest.raises(ValueError, match='Adjoint solver crash'):
+        solver.adjoint(simu, optim)
+    solver.release()
