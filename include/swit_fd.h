@@ -111,13 +111,15 @@ const char *swit_last_error(void);
 /* Plan options (no counterpart in the reference: tuning and measurement of the
  * batched path). */
 #define SWIT_OPT_GRAPH 1        /* 1 (default): replay the time loop as a hipGraph    */
-#define SWIT_OPT_PROFILE 2      /* 1: time every step kernel with HIP events          */
+#define SWIT_OPT_PROFILE 2      /* 1: time each step-kernel time loop with HIP events */
 #define SWIT_OPT_FWD_GROUP 4    /* shots per workgroup of the forward step kernel     */
 int swit_plan_set_option(swit_plan *plan, int option, int value);
 
 /* Accumulated duration and launch count of the step kernels timed while
- * SWIT_OPT_PROFILE was on: which = 0 forward step kernel, 1 backward step kernel. */
-int swit_plan_kernel_time(const swit_plan *plan, int which, double *total_ms, int64_t *launches);
+ * SWIT_OPT_PROFILE was on: which = 0 forward step kernel, 1 backward step kernel.
+ * Each time loop is bracketed by one pair of events (the step kernels run back to
+ * back in between); this call waits for the loops recorded so far to finish. */
+int swit_plan_kernel_time(swit_plan *plan, int which, double *total_ms, int64_t *launches);
 
 /* Bytes of device scratch the plan holds (wavefields, strips, accumulators). */
 int64_t swit_plan_scratch_bytes(const swit_plan *plan);

@@ -677,8 +677,10 @@ struct BatchedState {
     hipGraphExec_t gfwd[2] = {nullptr, nullptr};
     hipGraphExec_t gbwd = nullptr;
     int use_graph = 1;
-    int profile = 0;             // record events around the step kernels
-    std::vector<hipEvent_t> ev;  // [2 * nt] start/stop pairs
+    int profile = 0;             // bracket each time loop with a pair of events
+    struct Timed { int which; long long launches; hipEvent_t e0, e1; };
+    std::vector<Timed> pending;  // recorded, not yet read
+    std::vector<hipEvent_t> ev_pool;
     double prof_ms[2] = {0, 0};  // accumulated fwd / bwd step-kernel time
     long long prof_n[2] = {0, 0};
 };

@@ -575,8 +575,10 @@ static void batched_release(swit_plan *pl) {
     for (auto &g : bs.gfwd)
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
     if (bs.gbwd) { (void)hipGraphExecDestroy(bs.gbwd); bs.gbwd = nullptr; }
-    for (hipEvent_t e : bs.ev) (void)hipEventDestroy(e);
-    bs.ev.clear();
+    for (auto &t : bs.pending) { (void)hipEventDestroy(t.e0); (void)hipEventDestroy(t.e1); }
+    bs.pending.clear();
+    for (hipEvent_t e : bs.ev_pool) (void)hipEventDestroy(e);
+    bs.ev_pool.clear();
     if (bs.cap) { (void)hipStreamDestroy(bs.cap); bs.cap = nullptr; }
     bs.ready = 0;
 }
@@ -589,17 +591,34 @@ static int batched_set_model(swit_plan *pl, hipStream_t st) {
     return SWIT_OK;
 }
 
-static int ensure_events(BatchedState &bs, int n) {
-    while ((int)bs.ev.size() < n) {
-        hipEvent_t e;
-        HIP_TRY(hipEventCreate(&e));
-        bs.ev.push_back(e);
+static int pool_event(BatchedState &bs, hipEvent_t *e) {
+    if (!bs.ev_pool.empty()) {
+        *e = bs.ev_pool.back();
+        bs.ev_pool.pop_back();
+        return SWIT_OK;
     }
+    HIP_TRY(hipEventCreate(e));
     return SWIT_OK;
 }
 
-// Enqueue the forward time loop on stream st (captured into a graph or run directly).
-static int enqueue_forward(swit_plan *pl, int save, hipStream_t st, bool timed) {
+// Read every recorded loop timing (waits for its end event) into prof_ms / prof_n.
+static int drain_timings(BatchedState &bs) {
+    for (auto &t : bs.pending) {
+        float ms = 0;
+        HIP_TRY(hipEventSynchronize(t.e1));
+        HIP_TRY(hipEventElapsedTime(&ms, t.e0, t.e1));
+        bs.prof_ms[t.which] += ms;
+        bs.prof_n[t.which] += t.launches;
+        bs.ev_pool.push_back(t.e0);
+        bs.ev_pool.push_back(t.e1);
+    }
+    bs.pending.clear();
+    return SWIT_OK;
+}
+
+// Forward time loop: prologue (state clear, zero first rows), the nt-1 step kernels,
+// epilogue (final state for the reconstruction).
+static int forward_prologue(swit_plan *pl, int save, hipStream_t st) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
     const int ns = pl->ns;
@@ -612,26 +631,37 @@ static int enqueue_forward(swit_plan *pl, int save, hipStream_t st, bool timed) 
         hipLaunchKernelGGL(k_zero_rows, dim3(ns), dim3(256), 0, st, pl->strips, (size_t)d.nt * d.S, d.S);
         LAUNCH_CHECK();
     }
+    return SWIT_OK;
+}
+
+static int forward_loop(swit_plan *pl, int save, hipStream_t st) {
+    BatchedState &bs = pl->bat;
+    const Dims &d = pl->d;
     const int nblk = bs.ngroups_f * bs.fwd.ntiles;
     for (int it = 2; it <= d.nt; ++it) {
         const float *in = bs.F[it % 2];          // it = 2 reads F[0]
         float *out = bs.F[(it + 1) % 2];
-        if (timed) HIP_TRY(hipEventRecord(bs.ev[2 * (it - 2)], st));
-        hipLaunchKernelGGL(k_fwd_fused, dim3(nblk), dim3(kThreads), 0, st, d, it, bs.fwd, ns, bs.Gf, bs.CP, bs.CV,
-                           pl->geo, bs.src, in, out, bs.rec_start, bs.rec_list, pl->seis_t, bs.strip_start,
-                           bs.strip_list, save ? pl->strips : nullptr);
-        LAUNCH_CHECK();
-        if (timed) HIP_TRY(hipEventRecord(bs.ev[2 * (it - 2) + 1], st));
-    }
-    if (save) {
-        const float *fin = bs.F[(d.nt + 1) % 2];
-        hipLaunchKernelGGL(k_end_state, dim3(cdiv(d.nz, 64), cdiv(d.nx, 4), ns), dim3(64, 4), 0, st, d, fin, bs.end);
+        hipLaunchKernelGGL(k_fwd_fused, dim3(nblk), dim3(kThreads), 0, st, d, it, bs.fwd, pl->ns, bs.Gf, bs.CP,
+                           bs.CV, pl->geo, bs.src, in, out, bs.rec_start, bs.rec_list, pl->seis_t,
+                           bs.strip_start, bs.strip_list, save ? pl->strips : nullptr);
         LAUNCH_CHECK();
     }
     return SWIT_OK;
 }
 
-static int enqueue_adjoint(swit_plan *pl, hipStream_t st, bool timed) {
+static int forward_epilogue(swit_plan *pl, int save, hipStream_t st) {
+    BatchedState &bs = pl->bat;
+    const Dims &d = pl->d;
+    if (save) {
+        const float *fin = bs.F[(d.nt + 1) % 2];
+        hipLaunchKernelGGL(k_end_state, dim3(cdiv(d.nz, 64), cdiv(d.nx, 4), pl->ns), dim3(64, 4), 0, st, d, fin,
+                           bs.end);
+        LAUNCH_CHECK();
+    }
+    return SWIT_OK;
+}
+
+static int adjoint_prologue(swit_plan *pl, hipStream_t st) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
     const int ns = pl->ns;
@@ -639,29 +669,41 @@ static int enqueue_adjoint(swit_plan *pl, hipStream_t st, bool timed) {
     TRY(fill(bs.Bk[0], 3 * np * ns, st));
     TRY(copyk(bs.end, bs.R[0], 3 * ni * ns, st));
     TRY(fill(bs.acc, 3 * ni * ns, st));
-    const int nblk = ns * bs.bwd.ntiles;
+    return SWIT_OK;
+}
+
+static int adjoint_loop(swit_plan *pl, hipStream_t st) {
+    BatchedState &bs = pl->bat;
+    const Dims &d = pl->d;
+    const int nblk = pl->ns * bs.bwd.ntiles;
     for (int it = d.nt; it >= 2; --it) {
         int k = d.nt - it;
         const float *ri = bs.R[k % 2], *bi = bs.Bk[k % 2];
         float *ro = bs.R[(k + 1) % 2], *bo = bs.Bk[(k + 1) % 2];
-        if (timed) HIP_TRY(hipEventRecord(bs.ev[2 * k], st));
         hipLaunchKernelGGL(k_bwd_fused, dim3(nblk), dim3(kThreads), 0, st, d, it, bs.bwd, bs.CP, bs.CV,
                            pl->geo, bs.src, pl->strips, ri, ro, bi, bo, bs.inj_start, bs.inj_groups,
                            bs.inj_list, pl->sadj_t, bs.acc);
         LAUNCH_CHECK();
-        if (timed) HIP_TRY(hipEventRecord(bs.ev[2 * k + 1], st));
     }
-    hipLaunchKernelGGL(k_reduce_acc, dim3(cdiv((int)ni, 256)), dim3(256), 0, st, d, ns, pl->c_int, bs.acc,
+    return SWIT_OK;
+}
+
+static int adjoint_epilogue(swit_plan *pl, hipStream_t st) {
+    BatchedState &bs = pl->bat;
+    const Dims &d = pl->d;
+    const size_t ni = (size_t)d.nx * d.nz;
+    hipLaunchKernelGGL(k_reduce_acc, dim3(cdiv((int)ni, 256)), dim3(256), 0, st, d, pl->ns, pl->c_int, bs.acc,
                        bs.out, bs.out + ni, bs.out + 2 * ni);
     LAUNCH_CHECK();
     return SWIT_OK;
 }
 
+// Capture one time loop (which = 0 forward, 1 backward) into an executable graph.
 static int capture(swit_plan *pl, hipGraphExec_t *exec, int which, int save) {
     BatchedState &bs = pl->bat;
     hipGraph_t graph = nullptr;
     HIP_TRY(hipStreamBeginCapture(bs.cap, hipStreamCaptureModeThreadLocal));
-    int rc = which == 0 ? enqueue_forward(pl, save, bs.cap, false) : enqueue_adjoint(pl, bs.cap, false);
+    int rc = which == 0 ? forward_loop(pl, save, bs.cap) : adjoint_loop(pl, bs.cap);
     hipError_t e = hipStreamEndCapture(bs.cap, &graph);
     if (rc != SWIT_OK) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -674,17 +716,26 @@ static int capture(swit_plan *pl, hipGraphExec_t *exec, int which, int save) {
     return SWIT_OK;
 }
 
-static int collect_profile(swit_plan *pl, int which, hipStream_t st) {
+// Run one time loop on st: the captured graph, or direct launches.  With profiling on,
+// a pair of events brackets the loop; the step kernels run back to back in between,
+// so the pair's elapsed time / (nt - 1) is the step kernel's average duration.
+static int run_loop(swit_plan *pl, int which, int save, hipStream_t st) {
     BatchedState &bs = pl->bat;
-    HIP_TRY(hipStreamSynchronize(st));
-    double tot = 0;
-    for (int k = 0; k < pl->d.nt - 1; ++k) {
-        float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, bs.ev[2 * k], bs.ev[2 * k + 1]));
-        tot += ms;
+    BatchedState::Timed t{which, (long long)pl->d.nt - 1, nullptr, nullptr};
+    if (bs.profile) {
+        TRY(pool_event(bs, &t.e0));
+        TRY(pool_event(bs, &t.e1));
+        bs.pending.push_back(t);
+        HIP_TRY(hipEventRecord(t.e0, st));
     }
-    bs.prof_ms[which] += tot;
-    bs.prof_n[which] += pl->d.nt - 1;
+    if (bs.use_graph) {
+        hipGraphExec_t &g = which == 0 ? bs.gfwd[save ? 1 : 0] : bs.gbwd;
+        if (!g) TRY(capture(pl, &g, which, save));
+        HIP_TRY(hipGraphLaunch(g, st));
+    } else {
+        TRY(which == 0 ? forward_loop(pl, save, st) : adjoint_loop(pl, st));
+    }
+    if (bs.profile) HIP_TRY(hipEventRecord(t.e1, st));
     return SWIT_OK;
 }
 
@@ -692,16 +743,9 @@ static int batched_forward(swit_plan *pl, const float *src, int save, hipStream_
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
     HIP_TRY(hipMemcpyAsync(bs.src, src, sizeof(float) * pl->ns * d.nt, hipMemcpyDeviceToDevice, st));
-    if (bs.profile) {
-        TRY(ensure_events(bs, 2 * (d.nt - 1)));
-        TRY(enqueue_forward(pl, save, st, true));
-        return collect_profile(pl, 0, st);
-    }
-    if (!bs.use_graph) return enqueue_forward(pl, save, st, false);
-    hipGraphExec_t &g = bs.gfwd[save ? 1 : 0];
-    if (!g) TRY(capture(pl, &g, 0, save));
-    HIP_TRY(hipGraphLaunch(g, st));
-    return SWIT_OK;
+    TRY(forward_prologue(pl, save, st));
+    TRY(run_loop(pl, 0, save, st));
+    return forward_epilogue(pl, save, st);
 }
 
 static int batched_adjoint(swit_plan *pl, const float *src, double *grad, double *fw, double *bk,
@@ -710,16 +754,9 @@ static int batched_adjoint(swit_plan *pl, const float *src, double *grad, double
     const Dims &d = pl->d;
     const size_t ni = (size_t)d.nx * d.nz;
     HIP_TRY(hipMemcpyAsync(bs.src, src, sizeof(float) * pl->ns * d.nt, hipMemcpyDeviceToDevice, st));
-    if (bs.profile) {
-        TRY(ensure_events(bs, 2 * (d.nt - 1)));
-        TRY(enqueue_adjoint(pl, st, true));
-        TRY(collect_profile(pl, 1, st));
-    } else if (!bs.use_graph) {
-        TRY(enqueue_adjoint(pl, st, false));
-    } else {
-        if (!bs.gbwd) TRY(capture(pl, &bs.gbwd, 1, 0));
-        HIP_TRY(hipGraphLaunch(bs.gbwd, st));
-    }
+    TRY(adjoint_prologue(pl, st));
+    TRY(run_loop(pl, 1, 0, st));
+    TRY(adjoint_epilogue(pl, st));
     HIP_TRY(hipMemcpyAsync(grad, bs.out, sizeof(double) * ni, hipMemcpyDeviceToDevice, st));
     HIP_TRY(hipMemcpyAsync(fw, bs.out + ni, sizeof(double) * ni, hipMemcpyDeviceToDevice, st));
     HIP_TRY(hipMemcpyAsync(bk, bs.out + 2 * ni, sizeof(double) * ni, hipMemcpyDeviceToDevice, st));
@@ -738,7 +775,11 @@ extern "C" int swit_plan_set_option(swit_plan *pl, int option, int value) {
         return SWIT_OK;
     case SWIT_OPT_PROFILE:
         bs.profile = value != 0;
-        if (value) { bs.prof_ms[0] = bs.prof_ms[1] = 0; bs.prof_n[0] = bs.prof_n[1] = 0; }
+        if (value) {
+            TRY(drain_timings(bs));
+            bs.prof_ms[0] = bs.prof_ms[1] = 0;
+            bs.prof_n[0] = bs.prof_n[1] = 0;
+        }
         return SWIT_OK;
     case SWIT_OPT_FWD_GROUP:
         if (value < 1 || value > pl->ns) return set_err(SWIT_EINVAL, "group size %d outside [1, ns]", value);
@@ -752,9 +793,10 @@ extern "C" int swit_plan_set_option(swit_plan *pl, int option, int value) {
     }
 }
 
-extern "C" int swit_plan_kernel_time(const swit_plan *pl, int which, double *total_ms, int64_t *launches) {
+extern "C" int swit_plan_kernel_time(swit_plan *pl, int which, double *total_ms, int64_t *launches) {
     g_err[0] = 0;
     if (!pl || which < 0 || which > 1 || !total_ms || !launches) return set_err(SWIT_EINVAL, "bad arguments");
+    TRY(drain_timings(pl->bat));
     *total_ms = pl->bat.prof_ms[which];
     *launches = pl->bat.prof_n[which];
     return SWIT_OK;

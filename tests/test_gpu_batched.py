@@ -83,16 +83,30 @@ def test_adjoint_needs_saved_strips():
 
 
 @pytest.mark.gpu
-def test_profile_counts_every_step():
+@pytest.mark.parametrize("graph", [1, 0])
+def test_profile_counts_every_step(graph):
+    """Profiling counts nt-1 launches per time loop, accumulates over calls, restarts
+    from zero when switched on again, and leaves the results unchanged."""
     w, src, adj, f, a = _case(seed=14, ns=2, nt=150)
     p = _plan(w)
+    p.set_option("graph", graph)
+    seis0, g0 = _run(p, src, adj)
     p.set_option("profile", 1)
+    seis1, g1 = _run(p, src, adj)
     _run(p, src, adj)
     fm, fn = p.kernel_time("forward")
     bm, bn = p.kernel_time("backward")
+    p.set_option("profile", 0)
+    _run(p, src, adj)
+    assert p.kernel_time("forward")[1] == fn
+    p.set_option("profile", 1)
+    assert p.kernel_time("backward") == (0.0, 0)
     p.close()
-    assert fn == w.nt - 1 and bn == w.nt - 1
+    assert fn == 2 * (w.nt - 1) and bn == 2 * (w.nt - 1)
     assert fm > 0 and bm > 0
+    np.testing.assert_array_equal(seis1, seis0)
+    for x, y in zip(g1, g0):
+        np.testing.assert_array_equal(x, y)
 
 
 @pytest.mark.gpu
