@@ -121,6 +121,12 @@ int swit_plan_set_option(swit_plan *plan, int option, int value);
  * back in between); this call waits for the loops recorded so far to finish. */
 int swit_plan_kernel_time(swit_plan *plan, int which, double *total_ms, int64_t *launches);
 
+/* The saved boundary strips of the last strip-saving forward, decoded to FP32:
+ * out[ns][nt][S] (device), S = 9nx+6nz, row it-1 = step it, the layout of the
+ * reference's boundary.bin per shot (fdcore.f90:130-159,166-176).
+ * SWIT_ESTATE if no strips are saved. */
+int swit_plan_strips(swit_plan *plan, float *out, hipStream_t stream);
+
 /* Bytes of device scratch the plan holds (wavefields, strips, accumulators). */
 int64_t swit_plan_scratch_bytes(const swit_plan *plan);
 

@@ -31,6 +31,7 @@ EXPORTS = (
     "swit_plan_destroy",
     "swit_last_error",
     "swit_plan_scratch_bytes",
+    "swit_plan_strips",
     "swit_plan_set_option",
     "swit_plan_kernel_time",
 )
@@ -99,6 +100,8 @@ def load(path: str | None = None):
     L.swit_plan_destroy.restype = ctypes.c_int
     L.swit_last_error.argtypes = []
     L.swit_last_error.restype = ctypes.c_char_p
+    L.swit_plan_strips.argtypes = [vp, vp, vp]
+    L.swit_plan_strips.restype = ctypes.c_int
     L.swit_plan_scratch_bytes.argtypes = [vp]
     L.swit_plan_scratch_bytes.restype = ctypes.c_int64
     L.swit_plan_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_int]

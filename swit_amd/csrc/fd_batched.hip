@@ -131,6 +131,83 @@ struct InjGroup {
 };
 
 
+// ---- 16-bit boundary strips ---------------------------------------------------------
+// Strip mode SM: 0 FP32 (the reference's values), 1 FP16, 2 BF16.  In the 16-bit modes
+// the values of one shot and step are scaled by 2^k, the power of two that puts the
+// step's max |v| over the shot's strip (m) into [2^14, 2^15): k = min(126, 141 - e(m))
+// with e(m) the biased FP32 exponent.  The scaling is exact, so the value is rounded
+// once (to nearest even) into FP16 or BF16; decoding multiplies by 2^-k (exact).  Only
+// m is stored per row.
+__device__ __forceinline__ int strip_shift(uint32_t mbits) { return min(126, 141 - (int)((mbits >> 23) & 0xffu)); }
+__device__ __forceinline__ float strip_inv(uint32_t mbits) {
+    return __uint_as_float((uint32_t)(strip_shift(mbits) + 127) << 23);
+}
+__device__ __forceinline__ float strip_scale(uint32_t mbits) {
+    return __uint_as_float((uint32_t)(127 - strip_shift(mbits)) << 23);
+}
+
+template <int SM> __device__ __forceinline__ uint16_t strip_encode(float v);
+template <> __device__ __forceinline__ uint16_t strip_encode<1>(float v) {
+    return __builtin_bit_cast(uint16_t, (_Float16)v);
+}
+template <> __device__ __forceinline__ uint16_t strip_encode<2>(float v) {
+    uint32_t b = __float_as_uint(v);
+    b += 0x7fffu + ((b >> 16) & 1u);          // round to nearest even (finite values)
+    return (uint16_t)(b >> 16);
+}
+template <int SM> __device__ __forceinline__ float strip_decode(uint16_t h);
+template <> __device__ __forceinline__ float strip_decode<1>(uint16_t h) {
+    return (float)__builtin_bit_cast(_Float16, h);
+}
+template <> __device__ __forceinline__ float strip_decode<2>(uint16_t h) {
+    return __uint_as_float((uint32_t)h << 16);
+}
+
+// One strip row (one shot, one step) as the backward kernel reads it.
+template <int SM> struct StripRow {
+    const uint16_t *b;
+    float scale;
+    __device__ __forceinline__ StripRow(const void *base, size_t row, int S, const uint32_t *smax)
+        : b(static_cast<const uint16_t *>(base) + row * S), scale(strip_scale(smax[row])) {}
+    __device__ __forceinline__ float operator[](int i) const { return strip_decode<SM>(b[i]) * scale; }
+};
+template <> struct StripRow<0> {
+    const float *b;
+    __device__ __forceinline__ StripRow(const void *base, size_t row, int S, const uint32_t *)
+        : b(static_cast<const float *>(base) + row * S) {}
+    __device__ __forceinline__ float operator[](int i) const { return b[i]; }
+};
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Encode the last step's strip row from the FP32 staging row (the forward step kernel
+// encodes each earlier row one step later).  grid (ceil(S/256), ns).
+template <int SM>
+__global__ void k_strip_flush(Dims d, int row, const float *__restrict__ stage, const uint32_t *__restrict__ smax,
+                              uint16_t *__restrict__ strips16) {
+    const int s = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.S) return;
+    const size_t r = (size_t)s * d.nt + row;
+    strips16[r * d.S + i] = strip_encode<SM>(stage[(size_t)s * d.S + i] * strip_inv(smax[r]));
+}
+
+// Decode 16-bit strip rows to FP32: n = ns * nt * S values, row r = index / S.
+template <int SM>
+__global__ void k_strip_decode(size_t n, int S, const uint16_t *__restrict__ strips16,
+                               const uint32_t *__restrict__ smax, float *__restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        const size_t r = i / (size_t)S;
+        out[i] = strip_decode<SM>(strips16[i]) * strip_scale(smax[r]);
+    }
+}
+
 // ---- small helpers (kernels only, so captured graphs hold no memset nodes) -------
 __global__ void k_fill(float *__restrict__ p, size_t n, float v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -146,6 +223,10 @@ __global__ void k_fill_d(double *__restrict__ p, size_t n) {
 __global__ void k_zero_rows(float *__restrict__ p, size_t stride, int width) {
     float *q = p + (size_t)blockIdx.x * stride;
     for (int i = threadIdx.x; i < width; i += blockDim.x) q[i] = 0.0f;
+}
+__global__ void k_zero_rows16(uint16_t *__restrict__ p, size_t stride, int width) {
+    uint16_t *q = p + (size_t)blockIdx.x * stride;
+    for (int i = threadIdx.x; i < width; i += blockDim.x) q[i] = 0;
 }
 __global__ void k_copy(const float *__restrict__ a, float *__restrict__ b, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -177,12 +258,15 @@ __global__ void k_setup_cvec(Dims d, const float *__restrict__ K, const float *_
 // ---------------------------------------------------------------------------------
 // Forward step it (fdcore.f90:75-160) for a group of G shots per workgroup.
 // In/Out: [ns][3][nxp][ld].  LDS window origin (zo, xo) = (z0-3, x0-3).
+// strips: null (no saving), FP32 rows (SM = 0) or 16-bit rows (SM = 1, 2) with the
+// FP32 staging rows stage[2][ns][S] and per-row maxima smax[ns][nt] (float bits).
+template <int SM>
 __global__ __launch_bounds__(kThreads) void k_fwd_fused(
     Dims d, int it, TileMap tm, int ns, int G, const float2 *__restrict__ CP, const float4 *__restrict__ CV,
     const ShotGeom *__restrict__ geo, const float *__restrict__ src, const float *__restrict__ In,
     float *__restrict__ Out, const int *__restrict__ rec_start, const RecRef *__restrict__ rec_list,
     float *__restrict__ seis_t, const int *__restrict__ strip_start, const StripRef *__restrict__ strip_list,
-    float *__restrict__ strips) {
+    void *__restrict__ strips, float *__restrict__ stage, uint32_t *__restrict__ smax) {
     using namespace fk;
     __shared__ float sP[kNR];
     __shared__ float sU[kNR];
@@ -327,11 +411,33 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
         // strips (fdcore.f90:130-159)
         if (strips) {
             int a0 = strip_start[t], a1 = strip_start[t + 1];
-            float *row = strips + ((size_t)s * d.nt + (it - 1)) * d.S;
-            for (int a = a0 + tid; a < a1; a += kThreads) {
-                StripRef sr = strip_list[a];
-                int l = L(sr.iz, sr.ix, zo, xo);
-                row[sr.slot] = sr.field == 0 ? sQ[l] : (sr.field == 1 ? sU[l] : sW[l]);
+            if constexpr (SM == 0) {
+                float *row = static_cast<float *>(strips) + ((size_t)s * d.nt + (it - 1)) * d.S;
+                for (int a = a0 + tid; a < a1; a += kThreads) {
+                    StripRef sr = strip_list[a];
+                    int l = L(sr.iz, sr.ix, zo, xo);
+                    row[sr.slot] = sr.field == 0 ? sQ[l] : (sr.field == 1 ? sU[l] : sW[l]);
+                }
+            } else if (a1 > a0) {
+                // stage this step's values, encode the previous step's (its max is final)
+                float *now = stage + ((size_t)(it & 1) * ns + s) * d.S;
+                const float *prev = stage + ((size_t)((it - 1) & 1) * ns + s) * d.S;
+                const bool enc = it >= 3;
+                const size_t rprev = (size_t)s * d.nt + (it - 2);
+                uint16_t *row16 = static_cast<uint16_t *>(strips) + rprev * d.S;
+                const float inv = enc ? strip_inv(smax[rprev]) : 0.0f;
+                float amax = 0.0f;
+                for (int a = a0 + tid; a < a1; a += kThreads) {
+                    StripRef sr = strip_list[a];
+                    int l = L(sr.iz, sr.ix, zo, xo);
+                    float v = sr.field == 0 ? sQ[l] : (sr.field == 1 ? sU[l] : sW[l]);
+                    now[sr.slot] = v;
+                    amax = fmaxf(amax, fabsf(v));
+                    if (enc) row16[sr.slot] = strip_encode<SM>(prev[sr.slot] * inv);
+                }
+                amax = wave_max(amax);
+                if ((tid & 63) == 0 && amax > 0.0f)
+                    atomicMax(&smax[(size_t)s * d.nt + (it - 1)], __float_as_uint(amax));
             }
         }
         __syncthreads();
@@ -346,9 +452,11 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
 // RIn/ROut: [ns][3][nx][nz]; BIn/BOut: [ns][3][nxp][ld].
 // acc: [ns][3][nx][nz] FP32 per-shot accumulators (kernel, forward illumination,
 // backward illumination), updated in the reference's order.
+template <int SM>
 __global__ __launch_bounds__(kThreads) void k_bwd_fused(
     Dims d, int it, TileMap tm, const float2 *__restrict__ CP, const float4 *__restrict__ CV,
-    const ShotGeom *__restrict__ geo, const float *__restrict__ src, const float *__restrict__ strips,
+    const ShotGeom *__restrict__ geo, const float *__restrict__ src, const void *__restrict__ strips,
+    const uint32_t *__restrict__ smax,
     const float *__restrict__ RIn, float *__restrict__ ROut, const float *__restrict__ BIn,
     float *__restrict__ BOut, const int *__restrict__ inj_gstart, const InjGroup *__restrict__ inj_groups,
     const RecRef *__restrict__ inj_list, const float *__restrict__ sadj_t, float *__restrict__ acc) {
@@ -375,7 +483,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
     const float *Kc = reinterpret_cast<const float *>(CP);   // K at 2*k, 1-kappa_p at 2*k+1
     const float *Bc = reinterpret_cast<const float *>(CV);   // B at 4*k
     const ShotGeom g = geo[s];
-    const float *b = strips + ((size_t)s * d.nt + (it - 2)) * d.S;
+    const StripRow<SM> b(strips, (size_t)s * d.nt + (it - 2), d.S, smax);
     // this thread's row
     const int z = zo + lane, jz = z - npml;
     const bool zin = z >= 1 && z <= d.nzp;
@@ -672,6 +780,10 @@ struct BatchedState {
     float *end = nullptr;        // saved final forward state [ns][3][nx][nz]
     float *src = nullptr;        // plan-owned copy of the source traces
     float *acc = nullptr;        // [ns][3][nx][nz] per-shot FP32 accumulators
+    int strip_mode = 0;          // 0 FP32, 1 FP16, 2 BF16
+    uint16_t *strips16 = nullptr;   // [ns][nt][S] 16-bit strips
+    uint32_t *smax = nullptr;    // [ns][nt] per-row max |v| (float bits)
+    float *stage = nullptr;      // [2][ns][S] FP32 staging rows
     double *out = nullptr;       // [3][nx][nz]
     hipStream_t cap = nullptr;   // capture stream
     hipGraphExec_t gfwd[2] = {nullptr, nullptr};

@@ -115,6 +115,9 @@ static int plan_build(swit_plan *pl, const swit_grid *g, const swit_shots *sh, i
     if (!sh->ng || !sh->src_xz || !sh->rec_xz) return set_err(SWIT_EINVAL, "shot arrays are required");
     if ((flags & SWIT_STRIPS_F16) && (flags & SWIT_STRIPS_BF16))
         return set_err(SWIT_EINVAL, "choose one strip precision");
+    const bool s16 = (flags & (SWIT_STRIPS_F16 | SWIT_STRIPS_BF16)) != 0;
+    const bool batched = (flags & SWIT_PATH_BATCHED) != 0;
+    if (s16 && !batched) return set_err(SWIT_EINVAL, "16-bit strips need the batched path");
 
     Dims &d = pl->d;
     d.nx = g->nx;
@@ -201,33 +204,33 @@ static int plan_build(swit_plan *pl, const swit_grid *g, const swit_shots *sh, i
     TRY(pl->alloc(&pl->prof, 1024));
     TRY(pl->alloc(&pl->vmin, 64));
     TRY(pl->alloc(&pl->c_int, ni));
-    TRY(pl->alloc(&pl->P, np * ns));
-    TRY(pl->alloc(&pl->U, np * ns));
-    TRY(pl->alloc(&pl->W, np * ns));
-    TRY(pl->alloc(&pl->Pb[0], np * ns));
-    TRY(pl->alloc(&pl->Pb[1], np * ns));
-    TRY(pl->alloc(&pl->Ub, np * ns));
-    TRY(pl->alloc(&pl->Wb, np * ns));
-    TRY(pl->alloc(&pl->Pr, ni * ns));
-    TRY(pl->alloc(&pl->Ur, ni * ns));
-    TRY(pl->alloc(&pl->Wr, ni * ns));
-    TRY(pl->alloc(&pl->pe, ni * ns));
-    TRY(pl->alloc(&pl->ue, ni * ns));
-    TRY(pl->alloc(&pl->we, ni * ns));
-    TRY(pl->alloc(&pl->strips, (size_t)ns * d.nt * d.S));
+    if (!batched) {
+        // per-shot wavefields of the plain path
+        TRY(pl->alloc(&pl->P, np * ns));
+        TRY(pl->alloc(&pl->U, np * ns));
+        TRY(pl->alloc(&pl->W, np * ns));
+        TRY(pl->alloc(&pl->Pb[0], np * ns));
+        TRY(pl->alloc(&pl->Pb[1], np * ns));
+        TRY(pl->alloc(&pl->Ub, np * ns));
+        TRY(pl->alloc(&pl->Wb, np * ns));
+        TRY(pl->alloc(&pl->Pr, ni * ns));
+        TRY(pl->alloc(&pl->Ur, ni * ns));
+        TRY(pl->alloc(&pl->Wr, ni * ns));
+        TRY(pl->alloc(&pl->pe, ni * ns));
+        TRY(pl->alloc(&pl->ue, ni * ns));
+        TRY(pl->alloc(&pl->we, ni * ns));
+        TRY(pl->alloc(&pl->dg, ni * ns));
+        TRY(pl->alloc(&pl->dfw, ni * ns));
+        TRY(pl->alloc(&pl->dbk, ni * ns));
+    }
+    if (!s16) TRY(pl->alloc(&pl->strips, (size_t)ns * d.nt * d.S));
     TRY(pl->alloc(&pl->seis_t, (size_t)ns * d.nt * ngmax));
     TRY(pl->alloc(&pl->sadj_t, (size_t)ns * d.nt * ngmax));
-    TRY(pl->alloc(&pl->dg, ni * ns));
-    TRY(pl->alloc(&pl->dfw, ni * ns));
-    TRY(pl->alloc(&pl->dbk, ni * ns));
     HIP_TRY(hipMemcpy(pl->geo, pl->geo_h.data(), sizeof(ShotGeom) * ns, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(pl->rec_fwd, rf.data(), sizeof(int) * rf.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(pl->rec_inj, ri.data(), sizeof(int) * ri.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(pl->strip_map, sm.data(), sizeof(uint32_t) * sm.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(pl->P, 0, np * ns * sizeof(float)));
-    HIP_TRY(hipMemset(pl->U, 0, np * ns * sizeof(float)));
-    HIP_TRY(hipMemset(pl->W, 0, np * ns * sizeof(float)));
-    if (flags & SWIT_PATH_BATCHED) TRY(batched_init(pl));
+    if (batched) TRY(batched_init(pl));
     return SWIT_OK;
 }
 
@@ -261,6 +264,26 @@ extern "C" int swit_plan_destroy(swit_plan *plan) {
 extern "C" const char *swit_last_error(void) { return g_err; }
 
 extern "C" int64_t swit_plan_scratch_bytes(const swit_plan *plan) { return plan ? plan->bytes : 0; }
+
+extern "C" int swit_plan_strips(swit_plan *pl, float *out, hipStream_t stream) {
+    g_err[0] = 0;
+    if (!pl || !out) return set_err(SWIT_EINVAL, "plan and out are required");
+    if (!pl->strips_valid) return set_err(SWIT_ESTATE, "no saved strips");
+    const Dims &d = pl->d;
+    const size_t n = (size_t)pl->ns * d.nt * d.S;
+    const BatchedState &bs = pl->bat;
+    if (bs.strip_mode == 0) {
+        HIP_TRY(hipMemcpyAsync(out, pl->strips, n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        return SWIT_OK;
+    }
+    int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
+    if (bs.strip_mode == 1)
+        hipLaunchKernelGGL(k_strip_decode<1>, dim3(blocks), dim3(256), 0, stream, n, d.S, bs.strips16, bs.smax, out);
+    else
+        hipLaunchKernelGGL(k_strip_decode<2>, dim3(blocks), dim3(256), 0, stream, n, d.S, bs.strips16, bs.smax, out);
+    LAUNCH_CHECK();
+    return SWIT_OK;
+}
 
 extern "C" int swit_plan_set_model(swit_plan *pl, const float *vp, const float *rho,
                                    hipStream_t stream) {
@@ -549,6 +572,12 @@ static int batched_init(swit_plan *pl) {
     TRY(pl->alloc(&bs.src, (size_t)ns * d.nt));
     TRY(pl->alloc(&bs.acc, (size_t)ns * 3 * ni));
     TRY(pl->alloc(&bs.out, 3 * ni));
+    bs.strip_mode = (pl->flags & SWIT_STRIPS_F16) ? 1 : (pl->flags & SWIT_STRIPS_BF16) ? 2 : 0;
+    if (bs.strip_mode) {
+        TRY(pl->alloc(&bs.strips16, (size_t)ns * d.nt * d.S));
+        TRY(pl->alloc(&bs.smax, (size_t)ns * d.nt));
+        TRY(pl->alloc(&bs.stage, (size_t)2 * ns * d.S));
+    }
     HIP_TRY(hipMemcpy(bs.rec_start, rs.data(), sizeof(int) * rs.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(bs.rec_list, rl.data(), sizeof(RecRef) * rl.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(bs.inj_start, is.data(), sizeof(int) * is.size(), hipMemcpyHostToDevice));
@@ -627,23 +656,49 @@ static int forward_prologue(swit_plan *pl, int save, hipStream_t st) {
     // row 0 of the seismograms and strips is zero (csg(1,:), boundary(:,1))
     hipLaunchKernelGGL(k_zero_rows, dim3(ns), dim3(256), 0, st, pl->seis_t, (size_t)d.nt * d.ngmax, d.ngmax);
     LAUNCH_CHECK();
-    if (save) {
+    if (save && bs.strip_mode == 0) {
         hipLaunchKernelGGL(k_zero_rows, dim3(ns), dim3(256), 0, st, pl->strips, (size_t)d.nt * d.S, d.S);
         LAUNCH_CHECK();
+    } else if (save) {
+        hipLaunchKernelGGL(k_zero_rows16, dim3(ns), dim3(256), 0, st, bs.strips16, (size_t)d.nt * d.S, d.S);
+        LAUNCH_CHECK();
+        TRY(fill(reinterpret_cast<float *>(bs.smax), (size_t)ns * d.nt, st));   // +0.0f bits are 0
     }
     return SWIT_OK;
+}
+
+template <int SM>
+static void launch_fwd(swit_plan *pl, int it, const float *in, float *out, int save, hipStream_t st) {
+    BatchedState &bs = pl->bat;
+    const Dims &d = pl->d;
+    void *strips = !save ? nullptr : SM == 0 ? static_cast<void *>(pl->strips) : static_cast<void *>(bs.strips16);
+    hipLaunchKernelGGL(k_fwd_fused<SM>, dim3(bs.ngroups_f * bs.fwd.ntiles), dim3(kThreads), 0, st, d, it, bs.fwd,
+                       pl->ns, bs.Gf, bs.CP, bs.CV, pl->geo, bs.src, in, out, bs.rec_start, bs.rec_list,
+                       pl->seis_t, bs.strip_start, bs.strip_list, strips, bs.stage, bs.smax);
+}
+
+template <int SM>
+static void launch_bwd(swit_plan *pl, int it, const float *ri, float *ro, const float *bi, float *bo,
+                       hipStream_t st) {
+    BatchedState &bs = pl->bat;
+    const Dims &d = pl->d;
+    const void *strips = SM == 0 ? static_cast<const void *>(pl->strips) : static_cast<const void *>(bs.strips16);
+    hipLaunchKernelGGL(k_bwd_fused<SM>, dim3(pl->ns * bs.bwd.ntiles), dim3(kThreads), 0, st, d, it, bs.bwd,
+                       bs.CP, bs.CV, pl->geo, bs.src, strips, bs.smax, ri, ro, bi, bo, bs.inj_start,
+                       bs.inj_groups, bs.inj_list, pl->sadj_t, bs.acc);
 }
 
 static int forward_loop(swit_plan *pl, int save, hipStream_t st) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
-    const int nblk = bs.ngroups_f * bs.fwd.ntiles;
     for (int it = 2; it <= d.nt; ++it) {
         const float *in = bs.F[it % 2];          // it = 2 reads F[0]
         float *out = bs.F[(it + 1) % 2];
-        hipLaunchKernelGGL(k_fwd_fused, dim3(nblk), dim3(kThreads), 0, st, d, it, bs.fwd, pl->ns, bs.Gf, bs.CP,
-                           bs.CV, pl->geo, bs.src, in, out, bs.rec_start, bs.rec_list, pl->seis_t,
-                           bs.strip_start, bs.strip_list, save ? pl->strips : nullptr);
+        switch (bs.strip_mode) {
+        case 0: launch_fwd<0>(pl, it, in, out, save, st); break;
+        case 1: launch_fwd<1>(pl, it, in, out, save, st); break;
+        default: launch_fwd<2>(pl, it, in, out, save, st); break;
+        }
         LAUNCH_CHECK();
     }
     return SWIT_OK;
@@ -657,6 +712,17 @@ static int forward_epilogue(swit_plan *pl, int save, hipStream_t st) {
         hipLaunchKernelGGL(k_end_state, dim3(cdiv(d.nz, 64), cdiv(d.nx, 4), pl->ns), dim3(64, 4), 0, st, d, fin,
                            bs.end);
         LAUNCH_CHECK();
+        if (bs.strip_mode) {
+            // the last row (step nt) is still staged; the step kernels encoded rows 1..nt-2
+            // one step late and row 0 (step 1) is zero
+            const float *stage = bs.stage + (size_t)(d.nt & 1) * pl->ns * d.S;
+            dim3 grd(cdiv(d.S, 256), pl->ns);
+            if (bs.strip_mode == 1)
+                hipLaunchKernelGGL(k_strip_flush<1>, grd, dim3(256), 0, st, d, d.nt - 1, stage, bs.smax, bs.strips16);
+            else
+                hipLaunchKernelGGL(k_strip_flush<2>, grd, dim3(256), 0, st, d, d.nt - 1, stage, bs.smax, bs.strips16);
+            LAUNCH_CHECK();
+        }
     }
     return SWIT_OK;
 }
@@ -675,14 +741,15 @@ static int adjoint_prologue(swit_plan *pl, hipStream_t st) {
 static int adjoint_loop(swit_plan *pl, hipStream_t st) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
-    const int nblk = pl->ns * bs.bwd.ntiles;
     for (int it = d.nt; it >= 2; --it) {
         int k = d.nt - it;
         const float *ri = bs.R[k % 2], *bi = bs.Bk[k % 2];
         float *ro = bs.R[(k + 1) % 2], *bo = bs.Bk[(k + 1) % 2];
-        hipLaunchKernelGGL(k_bwd_fused, dim3(nblk), dim3(kThreads), 0, st, d, it, bs.bwd, bs.CP, bs.CV,
-                           pl->geo, bs.src, pl->strips, ri, ro, bi, bo, bs.inj_start, bs.inj_groups,
-                           bs.inj_list, pl->sadj_t, bs.acc);
+        switch (bs.strip_mode) {
+        case 0: launch_bwd<0>(pl, it, ri, ro, bi, bo, st); break;
+        case 1: launch_bwd<1>(pl, it, ri, ro, bi, bo, st); break;
+        default: launch_bwd<2>(pl, it, ri, ro, bi, bo, st); break;
+        }
         LAUNCH_CHECK();
     }
     return SWIT_OK;

@@ -141,6 +141,13 @@ class Plan:
                                              self._stream()))
         return tuple(outs)
 
+    def saved_strips(self):
+        """Saved boundary strips decoded to FP32: (ns, nt, 9nx+6nz) on the device."""
+        torch = self._torch
+        out = torch.empty((self.ns, self.nt, 9 * self.nx + 6 * self.nz), dtype=torch.float32, device=self.device)
+        _native.check(self._lib.swit_plan_strips(self._h, ctypes.c_void_p(out.data_ptr()), self._stream()))
+        return out
+
     def close(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

@@ -42,9 +42,11 @@ def test_batched_matches_oracle(graph):
     p = _plan(w)
     p.set_option("graph", graph)
     seis, (g, fw, bk) = _run(p, src, adj)
+    strips = p.saved_strips().cpu().numpy()
     p.close()
     print(f"graph={graph}: seis max|d| {np.abs(seis - f['seis']).max():.3e}, grad relL2 {rel_l2(g, a['grad']):.3e}")
     np.testing.assert_array_equal(seis, f["seis"])
+    np.testing.assert_array_equal(strips, f["boundary"])
     # per-shot FP32 imaging in the reference's order: bit-identical per-shot kernels,
     # FP64 cross-shot sums in the same shot order as the oracle
     assert rel_l2(g, a["grad"]) <= 1e-12
