@@ -240,7 +240,7 @@ __global__ void k_copy(const float *__restrict__ a, float *__restrict__ b, size_
 // (:109), each rounded exactly as there.
 __global__ void k_setup_cvec(Dims d, const float *__restrict__ K, const float *__restrict__ B,
                              const float *__restrict__ prof, float2 *__restrict__ CP,
-                             float4 *__restrict__ CV) {
+                             float4 *__restrict__ CV, float2 *__restrict__ KB) {
     int iz = blockIdx.x * blockDim.x + threadIdx.x + 1;
     int ix = blockIdx.y * blockDim.y + threadIdx.y + 1;
     if (iz > d.nzp || ix > d.nxp) return;
@@ -253,6 +253,7 @@ __global__ void k_setup_cvec(Dims d, const float *__restrict__ K, const float *_
     float kw = 0.5f * (d0 + dzp) * d.dt;
     CP[k] = make_float2(K[k], 1.0f - kp);
     CV[k] = make_float4(B[k], 1.0f - ku, 1.0f - kw, 0.0f);
+    KB[k] = make_float2(K[k], B[k]);
 }
 
 // ---------------------------------------------------------------------------------
@@ -449,12 +450,17 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
 // step (fdcore.f90:293-375) and imaging (solver.f90:237-239); one shot per workgroup.
 // The reconstruction window and the adjoint window are staged together and their
 // velocity and pressure updates share barriers.
+// A thread owns window rows = its lane and the columns 1 + wave + 8j; its tile cells are
+// the owned cells inside the tile, so each cell's {K, B} is loaded once for both the
+// velocity and the pressure pass.  The damping factors (1 - kappa) are loaded only by
+// tiles whose updated cells reach a damped cell; elsewhere they are exactly 1.
 // RIn/ROut: [ns][3][nx][nz]; BIn/BOut: [ns][3][nxp][ld].
 // acc: [ns][3][nx][nz] FP32 per-shot accumulators (kernel, forward illumination,
 // backward illumination), updated in the reference's order.
 template <int SM>
 __global__ __launch_bounds__(kThreads) void k_bwd_fused(
-    Dims d, int it, TileMap tm, const float2 *__restrict__ CP, const float4 *__restrict__ CV,
+    Dims d, int it, TileMap tm, const float2 *__restrict__ KB, const float2 *__restrict__ CP,
+    const float4 *__restrict__ CV,
     const ShotGeom *__restrict__ geo, const float *__restrict__ src, const void *__restrict__ strips,
     const uint32_t *__restrict__ smax,
     const float *__restrict__ RIn, float *__restrict__ ROut, const float *__restrict__ BIn,
@@ -464,10 +470,9 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
     __shared__ float rP[kNL];
     __shared__ float rU[kNL];
     __shared__ float rW[kNL];
-    __shared__ float bP[kNL];
+    __shared__ float bP[kNL];     // old adjoint p, then the new one in place
     __shared__ float bU[kNL];
     __shared__ float bW[kNL];
-    __shared__ float bQ[kNL];
 
     int s, t;
     work_item(xcd_logical(blockIdx.x, gridDim.x), (int)(gridDim.x / tm.ntiles), tm, s, t);
@@ -480,8 +485,11 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
     const bool has_interior = max(tb.z0, npml + 1) <= min(tb.z1, npml + nz) &&
                               max(tb.x0, npml + 1) <= min(tb.x1, npml + nx);
     const int top0 = 3 * nx + 6 * nz;
-    const float *Kc = reinterpret_cast<const float *>(CP);   // K at 2*k, 1-kappa_p at 2*k+1
-    const float *Bc = reinterpret_cast<const float *>(CV);   // B at 4*k
+    // updated cells span rows z0-2..z0+TZ and columns x0-2..x1+1; the velocity factors
+    // also read damping at the next row / column
+    const bool damped = !(tb.z0 - 2 >= npml + 1 && tb.z0 + kTZ + 1 <= npml + nz && tb.x0 - 2 >= npml + 1 &&
+                          tb.x1 + 2 <= npml + nx);
+    const float *Pc = reinterpret_cast<const float *>(CP);   // 1-kappa_p at 2*k+1
     const ShotGeom g = geo[s];
     const StripRow<SM> b(strips, (size_t)s * d.nt + (it - 2), d.S, smax);
     // this thread's row
@@ -496,7 +504,6 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
     // column sets: staged c in [0, kCX); velocities c in [1, cv_end]; tile c in [3, ct_end]
     constexpr int kJS = (kCX + kWaves - 1) / kWaves;
     constexpr int kJV = (kTX + 3 + kWaves - 1) / kWaves;
-    constexpr int kJT = (kTX + kWaves - 1) / kWaves;
     const int cv_end = tb.x1 + 1 - xo;
     const int ct_end = tb.x1 - xo;
 
@@ -524,44 +531,35 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
             }
         }
     }
-    // velocity cells: vq = {B (reconstruction alpha), 1-kappa_u, 1-kappa_w}, vk = K
-    // (adjoint alpha), vsu / vsw = strip values inserted into the reconstructed u / w
-    float4 vq[kJV];
-    float vk[kJV], vsu[kJV], vsw[kJV];
+    // owned cells: kb = {K, B}, fac = {1-kappa_p, 1-kappa_u, 1-kappa_w}, vsu / vsw =
+    // strip values inserted into the reconstructed u / w; on tile cells tsp = strip value
+    // inserted into the reconstructed p, ta* = this shot's accumulators
+    float2 kb[kJV];
+    float fp[kJV], fu[kJV], fw[kJV], vsu[kJV], vsw[kJV];
+    float tsp[kJV], ta0[kJV], ta1[kJV], ta2[kJV];
+    float *a = acc + (size_t)s * 3 * ni;
 #pragma unroll
     for (int j = 0; j < kJV; ++j) {
-        int c = 1 + wave + kWaves * j;
-        int x = xo + c, jx = x - npml;
-        vq[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-        vk[j] = vsu[j] = vsw[j] = 0.f;
+        const int c = 1 + wave + kWaves * j;
+        const int x = xo + c, jx = x - npml;
+        kb[j] = make_float2(0.f, 0.f);
+        fp[j] = fu[j] = fw[j] = 1.0f;
+        vsu[j] = vsw[j] = tsp[j] = ta0[j] = ta1[j] = ta2[j] = 0.f;
         if (c > cv_end || !lane_v || !zin || x < 1 || x > d.nxp) continue;
         const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
-        bool au = zu && x >= 2 && x <= d.nxp - 2;
-        bool aw = zw && x >= 2 && x <= d.nxp - 1;
-        bool rc = rz && jx >= 1 && jx <= nx;
-        if (au || aw || rc) vq[j] = CV[k];
-        if (au || aw) vk[j] = Kc[2 * k];
+        kb[j] = KB[k];
+        if (damped) {
+            const float4 cv = CV[k];
+            fu[j] = cv.y;
+            fw[j] = cv.z;
+            fp[j] = Pc[2 * k + 1];
+        }
+        const bool rc = rz && jx >= 1 && jx <= nx;
         if (rc) {
             if (jx == 1 || jx == nx - 1) vsu[j] = b[3 * nx + 6 * (jz - 1) + (jx == 1 ? 2 : 5)];
             if (jz == nz - 1) vsw[j] = b[3 * (jx - 1) + 2];
         }
-    }
-    // tile cells: tk = {K (reconstruction alpha), 1-kappa_p}, tbv = B (adjoint alpha),
-    // tsp = strip value inserted into the reconstructed p, ta* = this shot's accumulators
-    float2 tk[kJT];
-    float tbv[kJT], tsp[kJT], ta0[kJT], ta1[kJT], ta2[kJT];
-    float *a = acc + (size_t)s * 3 * ni;
-#pragma unroll
-    for (int j = 0; j < kJT; ++j) {
-        int c = kHalo + wave + kWaves * j;
-        int x = xo + c, jx = x - npml;
-        tk[j] = make_float2(0.f, 0.f);
-        tbv[j] = tsp[j] = ta0[j] = ta1[j] = ta2[j] = 0.f;
-        if (c > ct_end || !lane_t) continue;
-        const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
-        tk[j] = CP[k];
-        if (zp && x >= 3 && x <= d.nxp - 2) tbv[j] = Bc[4 * k];
-        if (rz && jx >= 1 && jx <= nx) {
+        if (c >= kHalo && c <= ct_end && lane_t && rc) {
             const unsigned ki = (unsigned)((jx - 1) * nz + (jz - 1));
             ta0[j] = a[ki];
             ta1[j] = a[ni + ki];
@@ -614,15 +612,15 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
         if (rz && jx >= 1 && jx <= nx) {
             if (jx == 1 || jx == nx - 1) rU[l] = vsu[j];
             else if (jx <= nx - 2 && jz >= 2 && jz <= nz - 1)
-                rU[l] = v_undo(rU[l], vq[j].x, rP[l], rP[l + kZL], rP[l + 2 * kZL], rP[l - kZL]);
+                rU[l] = v_undo(rU[l], kb[j].y, rP[l], rP[l + kZL], rP[l + 2 * kZL], rP[l - kZL]);
             if (jz == nz - 1) rW[l] = vsw[j];
             else if (jx >= 2 && jx <= nx - 1 && jz >= 2 && jz <= nz - 2)
-                rW[l] = v_undo(rW[l], vq[j].x, rP[l], rP[l + 1], rP[l + 2], rP[l - 1]);
+                rW[l] = v_undo(rW[l], kb[j].y, rP[l], rP[l + 1], rP[l + 2], rP[l - 1]);
         }
         if (zu && x >= 2 && x <= d.nxp - 2)
-            bU[l] = v_update_f(bU[l], vk[j], vq[j].y, bP[l], bP[l + kZL], bP[l + 2 * kZL], bP[l - kZL]);
+            bU[l] = v_update_f(bU[l], kb[j].x, fu[j], bP[l], bP[l + kZL], bP[l + 2 * kZL], bP[l - kZL]);
         if (zw && x >= 2 && x <= d.nxp - 1)
-            bW[l] = v_update_f(bW[l], vk[j], vq[j].z, bP[l], bP[l + 1], bP[l + 2], bP[l - 1]);
+            bW[l] = v_update_f(bW[l], kb[j].x, fw[j], bP[l], bP[l + 1], bP[l + 2], bP[l - 1]);
     }
     __syncthreads();
     // adjoint w mirror (:329-333), columns 2..nxp-1
@@ -639,13 +637,14 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
     }
     __syncthreads();
     // ---- pressures: reconstruction p' (:250-287), adjoint p' (:336-345) ------------
-    float prn[kJT];
+    // the adjoint p is updated in place: in this pass a thread reads bP only at its own cells
+    float prn[kJV], pold[kJV];
     float *rout = ROut + (size_t)s * 3 * ni;
 #pragma unroll
-    for (int j = 0; j < kJT; ++j) {
-        prn[j] = 0.f;
-        int c = kHalo + wave + kWaves * j;
-        if (c > ct_end || !lane_t) continue;
+    for (int j = 0; j < kJV; ++j) {
+        prn[j] = pold[j] = 0.f;
+        int c = 1 + wave + kWaves * j;
+        if (c < kHalo || c > ct_end || !lane_t) continue;
         int x = xo + c, jx = x - npml;
         int l = LW(c, lane);
         if (rz && jx >= 1 && jx <= nx) {
@@ -657,7 +656,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
                 if (jz == g.isz_i && jx == g.isx_i && g.isx_i > 2 && g.isx_i < nx - 1 && g.isz_i > 3 &&
                     g.isz_i < nz - 1)
                     v = v - d.dt * ssub;
-                v = p_undo(v, tk[j].x, rU[l], rU[l - kZL], rU[l + kZL], rU[l - 2 * kZL],
+                v = p_undo(v, kb[j].x, rU[l], rU[l - kZL], rU[l + kZL], rU[l - 2 * kZL],
                            rW[l], rW[l - 1], rW[l + 1], rW[l - 2]);
             }
             const unsigned k = (unsigned)((jx - 1) * nz + (jz - 1));
@@ -667,27 +666,28 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
             prn[j] = v;
         }
         float v = bP[l];
+        pold[j] = v;
         if (zp && x >= 3 && x <= d.nxp - 2)
-            v = p_update_f(v, tbv[j], tk[j].y, bU[l], bU[l - kZL], bU[l + kZL], bU[l - 2 * kZL],
+            v = p_update_f(v, kb[j].y, fp[j], bU[l], bU[l - kZL], bU[l + kZL], bU[l - 2 * kZL],
                            bW[l], bW[l - 1], bW[l + 1], bW[l - 2]);
-        bQ[l] = v;
+        bP[l] = v;
     }
     __syncthreads();
     // adjoint-source injection (:348-367): one thread per cell, receivers sharing a cell
     // added in receiver order
     if (g0 + (int)threadIdx.x < g1) {
         int l = LW(ir0.ix - xo, ir0.iz - zo);
-        float v = bQ[l] + d.dt * iv0;
+        float v = bP[l] + d.dt * iv0;
         for (int e = ig0.e0 + 1; e < ig0.e1; ++e) v = v + d.dt * sa[el[e].idx];
-        bQ[l] = v;
+        bP[l] = v;
     }
     for (int q = g0 + (int)threadIdx.x + kThreads; q < g1; q += kThreads) {
         InjGroup gr = inj_groups[(size_t)s * d.ngmax + q];
         RecRef r0 = el[gr.e0];
         int l = LW(r0.ix - xo, r0.iz - zo);
-        float v = bQ[l];
+        float v = bP[l];
         for (int e = gr.e0; e < gr.e1; ++e) v = v + d.dt * sa[el[e].idx];
-        bQ[l] = v;
+        bP[l] = v;
     }
     __syncthreads();
     // unconditional p mirror (:370-373): rows fs-1 and fs+1 are rows of one tile
@@ -695,9 +695,9 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
         const int lm = fs - 1 - zo;
         if (lane == lm && lm >= kHalo && fs + 1 <= tb.z1) {
 #pragma unroll
-            for (int j = 0; j < kJT; ++j) {
-                int c = kHalo + wave + kWaves * j;
-                if (c <= ct_end) bQ[LW(c, lm)] = -bQ[LW(c, lm + 2)];
+            for (int j = 0; j < kJV; ++j) {
+                int c = 1 + wave + kWaves * j;
+                if (c >= kHalo && c <= ct_end) bP[LW(c, lm)] = -bP[LW(c, lm + 2)];
             }
         }
     }
@@ -705,20 +705,20 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
     // ---- write back; imaging dg += p_rec (p0 - p), dfw += p_rec^2, dbk += p^2 -------
     float *bout = BOut + (size_t)s * 3 * np;
 #pragma unroll
-    for (int j = 0; j < kJT; ++j) {
-        int c = kHalo + wave + kWaves * j;
-        if (c > ct_end || !lane_t) continue;
+    for (int j = 0; j < kJV; ++j) {
+        int c = 1 + wave + kWaves * j;
+        if (c < kHalo || c > ct_end || !lane_t) continue;
         int x = xo + c, jx = x - npml;
         int l = LW(c, lane);
             const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
-        float pb = bQ[l];
+        float pb = bP[l];
         bout[k] = pb;
         bout[np + k] = bU[l];
         bout[2 * np + k] = bW[l];
         if (rz && jx >= 1 && jx <= nx) {
             const unsigned ki = (unsigned)((jx - 1) * nz + (jz - 1));
             float pr = prn[j];
-            a[ki] = ta0[j] + pr * (bP[l] - pb);
+            a[ki] = ta0[j] + pr * (pold[j] - pb);
             a[ni + ki] = ta1[j] + pr * pr;
             a[2 * ni + ki] = ta2[j] + pb * pb;
         }
@@ -772,6 +772,7 @@ struct BatchedState {
     RecRef *inj_list = nullptr;  // [ns][ngmax]
     int *strip_start = nullptr;  // [fwd.ntiles+1]
     StripRef *strip_list = nullptr;
+    float2 *KB = nullptr;        // [nxp][ld] {K, B}
     float2 *CP = nullptr;        // [nxp][ld]
     float4 *CV = nullptr;        // [nxp][ld]
     float *F[2] = {nullptr, nullptr};   // forward state [ns][3][nxp][ld], ping-pong

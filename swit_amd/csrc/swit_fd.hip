@@ -561,6 +561,7 @@ static int batched_init(swit_plan *pl) {
     TRY(pl->alloc(&bs.inj_groups, ig.size()));
     TRY(pl->alloc(&bs.strip_start, ss.size()));
     TRY(pl->alloc(&bs.strip_list, sl.size()));
+    TRY(pl->alloc(&bs.KB, np));
     TRY(pl->alloc(&bs.CP, np));
     TRY(pl->alloc(&bs.CV, np));
     for (int k = 0; k < 2; ++k) {
@@ -615,7 +616,8 @@ static void batched_release(swit_plan *pl) {
 static int batched_set_model(swit_plan *pl, hipStream_t st) {
     const Dims &d = pl->d;
     dim3 blk(64, 4), grd(cdiv(d.nzp, 64), cdiv(d.nxp, 4), 1);
-    hipLaunchKernelGGL(k_setup_cvec, grd, blk, 0, st, d, pl->K, pl->B, pl->prof, pl->bat.CP, pl->bat.CV);
+    hipLaunchKernelGGL(k_setup_cvec, grd, blk, 0, st, d, pl->K, pl->B, pl->prof, pl->bat.CP, pl->bat.CV,
+                       pl->bat.KB);
     LAUNCH_CHECK();
     return SWIT_OK;
 }
@@ -684,7 +686,7 @@ static void launch_bwd(swit_plan *pl, int it, const float *ri, float *ro, const 
     const Dims &d = pl->d;
     const void *strips = SM == 0 ? static_cast<const void *>(pl->strips) : static_cast<const void *>(bs.strips16);
     hipLaunchKernelGGL(k_bwd_fused<SM>, dim3(pl->ns * bs.bwd.ntiles), dim3(kThreads), 0, st, d, it, bs.bwd,
-                       bs.CP, bs.CV, pl->geo, bs.src, strips, bs.smax, ri, ro, bi, bo, bs.inj_start,
+                       bs.KB, bs.CP, bs.CV, pl->geo, bs.src, strips, bs.smax, ri, ro, bi, bo, bs.inj_start,
                        bs.inj_groups, bs.inj_list, pl->sadj_t, bs.acc);
 }
 
