@@ -127,6 +127,40 @@ int swit_plan_kernel_time(swit_plan *plan, int which, double *total_ms, int64_t 
  * SWIT_ESTATE if no strips are saved. */
 int swit_plan_strips(swit_plan *plan, float *out, hipStream_t stream);
 
+/* ---- misfits and adjoint sources (toolbox/misfit.py) -------------------------------
+ * obs, syn, adj: [ntraces][nt] device arrays (a plan's [ns][ngmax][nt] seismograms
+ * flattened); work: [ntraces] doubles of device scratch (per-trace misfits on return);
+ * misfit: one double on the device.  A trace whose observed or synthetic norm is zero
+ * contributes nothing and gets a zero adjoint source (misfit.py:128-148 and peers).
+ *   SWIT_MISFIT_WAVEFORM: adj = syn - obs, misfit = sum sqrt(sum r^2 dt)
+ *     (misfit.py:128-148, :263-282);
+ *   SWIT_MISFIT_GLOBAL_CORRELATION: o = obs/|obs|, s = syn/|syn|, c = Pearson(s, o),
+ *     misfit = -sum c, adj = (s c - o) / |syn| (misfit.py:219-242, :338-355).
+ * Sums run in FP64; the sum over traces has a fixed order (reproducible). */
+#define SWIT_MISFIT_WAVEFORM 0
+#define SWIT_MISFIT_GLOBAL_CORRELATION 1
+int swit_misfit(int kind, int ntraces, int nt, float dt, const float *obs, const float *syn, float *adj,
+                double *work, double *misfit, hipStream_t stream);
+
+/* ---- plan groups: uneven grids in one launch ------------------------------------
+ * A group runs several batched plans (different grids, shot counts, record lengths,
+ * strip precisions) as one job: each group forward / adjoint is a single hipGraph
+ * launch holding every plan's whole time loop on its own branch, so plans with short
+ * records fill the GPU while long ones finish.  No counterpart in the reference (one
+ * fd2dmpi job per grid, toolbox/solver.py:24-92); results are those of
+ * swit_forward / swit_adjoint on each plan alone.  The arrays below hold one entry
+ * per plan, in the order of `plans`; each entry is what the single-plan call takes.
+ * Plans must not be used by other calls while a group call on them is in flight, and
+ * must outlive the group. */
+typedef struct swit_group swit_group;
+int swit_group_create(swit_plan *const *plans, int nplans, swit_group **out);
+int swit_group_forward(swit_group *group, const float *const *src, float *const *seis, int save_strips,
+                       hipStream_t stream);
+int swit_group_adjoint(swit_group *group, const float *const *src, const float *const *adjsrc,
+                       double *const *grad, double *const *illum_forw, double *const *illum_back,
+                       hipStream_t stream);
+int swit_group_destroy(swit_group *group);
+
 /* Bytes of device scratch the plan holds (wavefields, strips, accumulators). */
 int64_t swit_plan_scratch_bytes(const swit_plan *plan);
 

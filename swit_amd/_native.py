@@ -32,10 +32,17 @@ EXPORTS = (
     "swit_last_error",
     "swit_plan_scratch_bytes",
     "swit_plan_strips",
+    "swit_misfit",
+    "swit_group_create",
+    "swit_group_forward",
+    "swit_group_adjoint",
+    "swit_group_destroy",
     "swit_plan_set_option",
     "swit_plan_kernel_time",
 )
 
+SWIT_MISFIT_WAVEFORM = 0
+SWIT_MISFIT_GLOBAL_CORRELATION = 1
 SWIT_OPT_GRAPH = 1
 SWIT_OPT_PROFILE = 2
 SWIT_OPT_FWD_GROUP = 4
@@ -100,6 +107,17 @@ def load(path: str | None = None):
     L.swit_plan_destroy.restype = ctypes.c_int
     L.swit_last_error.argtypes = []
     L.swit_last_error.restype = ctypes.c_char_p
+    L.swit_misfit.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, vp, vp, vp, vp, vp, vp]
+    L.swit_misfit.restype = ctypes.c_int
+    pp = ctypes.POINTER(vp)
+    L.swit_group_create.argtypes = [pp, ctypes.c_int, pp]
+    L.swit_group_create.restype = ctypes.c_int
+    L.swit_group_forward.argtypes = [vp, pp, pp, ctypes.c_int, vp]
+    L.swit_group_forward.restype = ctypes.c_int
+    L.swit_group_adjoint.argtypes = [vp, pp, pp, pp, pp, pp, vp]
+    L.swit_group_adjoint.restype = ctypes.c_int
+    L.swit_group_destroy.argtypes = [vp]
+    L.swit_group_destroy.restype = ctypes.c_int
     L.swit_plan_strips.argtypes = [vp, vp, vp]
     L.swit_plan_strips.restype = ctypes.c_int
     L.swit_plan_scratch_bytes.argtypes = [vp]

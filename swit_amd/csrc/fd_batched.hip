@@ -790,6 +790,7 @@ struct BatchedState {
     hipGraphExec_t gfwd[2] = {nullptr, nullptr};
     hipGraphExec_t gbwd = nullptr;
     int use_graph = 1;
+    int gen = 0;                 // bumped when captured launch arguments change
     int profile = 0;             // bracket each time loop with a pair of events
     struct Timed { int which; long long launches; hipEvent_t e0, e1; };
     std::vector<Timed> pending;  // recorded, not yet read

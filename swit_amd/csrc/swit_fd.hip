@@ -17,6 +17,7 @@
 #include "fd_kernels.h"
 #include "fd_plain.hip"
 #include "fd_batched.hip"
+#include "fd_misfit.hip"
 
 using namespace swit;
 
@@ -854,6 +855,7 @@ extern "C" int swit_plan_set_option(swit_plan *pl, int option, int value) {
         if (value < 1 || value > pl->ns) return set_err(SWIT_EINVAL, "group size %d outside [1, ns]", value);
         bs.Gf = value;
         bs.ngroups_f = cdiv(pl->ns, value);
+        ++bs.gen;
         for (auto &gx : bs.gfwd)
             if (gx) { (void)hipGraphExecDestroy(gx); gx = nullptr; }
         return SWIT_OK;
@@ -868,5 +870,219 @@ extern "C" int swit_plan_kernel_time(swit_plan *pl, int which, double *total_ms,
     TRY(drain_timings(pl->bat));
     *total_ms = pl->bat.prof_ms[which];
     *launches = pl->bat.prof_n[which];
+    return SWIT_OK;
+}
+
+// ---- plan groups ------------------------------------------------------------------------
+
+struct swit_group {
+    std::vector<swit_plan *> plans;
+    hipStream_t cap = nullptr;               // capture origin
+    std::vector<hipStream_t> branch;         // one capture branch per plan
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraphExec_t gf[2] = {nullptr, nullptr};   // forward without / with strip saving
+    hipGraphExec_t gb = nullptr;
+    std::vector<int> gens_f[2], gens_b;          // plan generations at capture
+
+    void drop(hipGraphExec_t &g) {
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    }
+    void release() {
+        drop(gf[0]);
+        drop(gf[1]);
+        drop(gb);
+        for (hipStream_t st : branch) (void)hipStreamDestroy(st);
+        branch.clear();
+        for (hipEvent_t e : join) (void)hipEventDestroy(e);
+        join.clear();
+        if (fork) { (void)hipEventDestroy(fork); fork = nullptr; }
+        if (cap) { (void)hipStreamDestroy(cap); cap = nullptr; }
+    }
+    std::vector<int> gens() const {
+        std::vector<int> v;
+        for (swit_plan *p : plans) v.push_back(p->bat.gen);
+        return v;
+    }
+};
+
+// Capture every plan's whole job (which = 0 forward, 1 adjoint) on its own branch.
+static int group_capture(swit_group *gr, int which, int save, hipGraphExec_t *exec) {
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(gr->cap, hipStreamCaptureModeThreadLocal));
+    int rc = SWIT_OK;
+    hipError_t e = hipEventRecord(gr->fork, gr->cap);
+    for (size_t i = 0; i < gr->plans.size() && rc == SWIT_OK && e == hipSuccess; ++i) {
+        swit_plan *pl = gr->plans[i];
+        hipStream_t st = gr->branch[i];
+        e = hipStreamWaitEvent(st, gr->fork, 0);
+        if (e != hipSuccess) break;
+        if (which == 0) {
+            rc = forward_prologue(pl, save, st);
+            if (rc == SWIT_OK) rc = forward_loop(pl, save, st);
+            if (rc == SWIT_OK) rc = forward_epilogue(pl, save, st);
+        } else {
+            rc = adjoint_prologue(pl, st);
+            if (rc == SWIT_OK) rc = adjoint_loop(pl, st);
+            if (rc == SWIT_OK) rc = adjoint_epilogue(pl, st);
+        }
+        if (rc == SWIT_OK) e = hipEventRecord(gr->join[i], st);
+        if (rc == SWIT_OK && e == hipSuccess) e = hipStreamWaitEvent(gr->cap, gr->join[i], 0);
+    }
+    hipError_t ee = hipStreamEndCapture(gr->cap, &graph);
+    if (rc != SWIT_OK || e != hipSuccess || ee != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc != SWIT_OK) return rc;
+        return set_err(SWIT_EHIP, "group capture: %s", hipGetErrorString(e != hipSuccess ? e : ee));
+    }
+    e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return set_err(SWIT_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    return SWIT_OK;
+}
+
+extern "C" int swit_group_create(swit_plan *const *plans, int nplans, swit_group **out) {
+    g_err[0] = 0;
+    if (!out) return set_err(SWIT_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!plans || nplans < 1) return set_err(SWIT_EINVAL, "need at least one plan");
+    swit_group *gr = new (std::nothrow) swit_group();
+    if (!gr) return set_err(SWIT_ENOMEM, "host allocation failed");
+    for (int i = 0; i < nplans; ++i) {
+        swit_plan *pl = plans[i];
+        if (!pl || !(pl->flags & SWIT_PATH_BATCHED)) {
+            delete gr;
+            return set_err(SWIT_EINVAL, "plan %d is not a batched plan", i);
+        }
+        for (swit_plan *q : gr->plans)
+            if (q == pl) {
+                delete gr;
+                return set_err(SWIT_EINVAL, "plan %d appears twice", i);
+            }
+        gr->plans.push_back(pl);
+    }
+    auto fail = [&](hipError_t e) {
+        gr->release();
+        delete gr;
+        return set_err(SWIT_EHIP, "group setup: %s", hipGetErrorString(e));
+    };
+    hipError_t e = hipStreamCreateWithFlags(&gr->cap, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(e);
+    e = hipEventCreateWithFlags(&gr->fork, hipEventDisableTiming);
+    if (e != hipSuccess) return fail(e);
+    for (int i = 0; i < nplans; ++i) {
+        hipStream_t st;
+        hipEvent_t ev;
+        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e != hipSuccess) return fail(e);
+        gr->branch.push_back(st);
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) return fail(e);
+        gr->join.push_back(ev);
+    }
+    *out = gr;
+    return SWIT_OK;
+}
+
+extern "C" int swit_group_destroy(swit_group *gr) {
+    if (!gr) return SWIT_OK;
+    (void)hipDeviceSynchronize();
+    gr->release();
+    delete gr;
+    return SWIT_OK;
+}
+
+extern "C" int swit_group_forward(swit_group *gr, const float *const *src, float *const *seis, int save_strips,
+                                  hipStream_t stream) {
+    g_err[0] = 0;
+    if (!gr || !src || !seis) return set_err(SWIT_EINVAL, "group, src and seis are required");
+    if (save_strips < 0 || save_strips > 2) return set_err(SWIT_EINVAL, "save_strips must be 0, 1 or 2");
+    const int save = save_strips == 1;
+    const size_t n = gr->plans.size();
+    for (size_t i = 0; i < n; ++i) {
+        if (!src[i] || !seis[i]) return set_err(SWIT_EINVAL, "plan %zu: src and seis are required", i);
+        if (!gr->plans[i]->model_set) return set_err(SWIT_ESTATE, "plan %zu: no model set", i);
+    }
+    for (size_t i = 0; i < n; ++i) {
+        swit_plan *pl = gr->plans[i];
+        if (save_strips != 2) pl->strips_valid = false;
+        HIP_TRY(hipMemcpyAsync(pl->bat.src, src[i], sizeof(float) * pl->ns * pl->d.nt, hipMemcpyDeviceToDevice,
+                               stream));
+    }
+    hipGraphExec_t &g = gr->gf[save];
+    if (g && gr->gens_f[save] != gr->gens()) gr->drop(g);
+    if (!g) {
+        TRY(group_capture(gr, 0, save, &g));
+        gr->gens_f[save] = gr->gens();
+    }
+    HIP_TRY(hipGraphLaunch(g, stream));
+    for (size_t i = 0; i < n; ++i) {
+        swit_plan *pl = gr->plans[i];
+        const Dims &d = pl->d;
+        hipLaunchKernelGGL(k_transpose, dim3(cdiv(d.ngmax, 32), cdiv(d.nt, 32), pl->ns), dim3(32, 8), 0, stream,
+                           d.nt, d.ngmax, pl->seis_t, seis[i]);
+        LAUNCH_CHECK();
+        if (save_strips != 2) pl->strips_valid = save_strips == 1;
+    }
+    return SWIT_OK;
+}
+
+extern "C" int swit_group_adjoint(swit_group *gr, const float *const *src, const float *const *adjsrc,
+                                  double *const *grad, double *const *illum_forw, double *const *illum_back,
+                                  hipStream_t stream) {
+    g_err[0] = 0;
+    if (!gr || !src || !adjsrc || !grad || !illum_forw || !illum_back)
+        return set_err(SWIT_EINVAL, "group, src, adjsrc and the three outputs are required");
+    const size_t n = gr->plans.size();
+    for (size_t i = 0; i < n; ++i) {
+        if (!src[i] || !adjsrc[i] || !grad[i] || !illum_forw[i] || !illum_back[i])
+            return set_err(SWIT_EINVAL, "plan %zu: missing array", i);
+        if (!gr->plans[i]->strips_valid)
+            return set_err(SWIT_ESTATE, "plan %zu: no saved strips: run a strip-saving forward first", i);
+    }
+    for (size_t i = 0; i < n; ++i) {
+        swit_plan *pl = gr->plans[i];
+        const Dims &d = pl->d;
+        hipLaunchKernelGGL(k_transpose, dim3(cdiv(d.nt, 32), cdiv(d.ngmax, 32), pl->ns), dim3(32, 8), 0, stream,
+                           d.ngmax, d.nt, adjsrc[i], pl->sadj_t);
+        LAUNCH_CHECK();
+        HIP_TRY(hipMemcpyAsync(pl->bat.src, src[i], sizeof(float) * pl->ns * d.nt, hipMemcpyDeviceToDevice,
+                               stream));
+    }
+    if (gr->gb && gr->gens_b != gr->gens()) gr->drop(gr->gb);
+    if (!gr->gb) {
+        TRY(group_capture(gr, 1, 0, &gr->gb));
+        gr->gens_b = gr->gens();
+    }
+    HIP_TRY(hipGraphLaunch(gr->gb, stream));
+    for (size_t i = 0; i < n; ++i) {
+        swit_plan *pl = gr->plans[i];
+        const size_t ni = (size_t)pl->d.nx * pl->d.nz;
+        const double *o = pl->bat.out;
+        HIP_TRY(hipMemcpyAsync(grad[i], o, sizeof(double) * ni, hipMemcpyDeviceToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(illum_forw[i], o + ni, sizeof(double) * ni, hipMemcpyDeviceToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(illum_back[i], o + 2 * ni, sizeof(double) * ni, hipMemcpyDeviceToDevice, stream));
+    }
+    return SWIT_OK;
+}
+
+// ---- misfits -------------------------------------------------------------------------------
+
+extern "C" int swit_misfit(int kind, int ntraces, int nt, float dt, const float *obs, const float *syn,
+                           float *adj, double *work, double *misfit, hipStream_t stream) {
+    g_err[0] = 0;
+    if (kind != SWIT_MISFIT_WAVEFORM && kind != SWIT_MISFIT_GLOBAL_CORRELATION)
+        return set_err(SWIT_EINVAL, "unknown misfit kind %d", kind);
+    if (ntraces < 1 || nt < 1) return set_err(SWIT_EINVAL, "need ntraces >= 1 and nt >= 1");
+    if (!obs || !syn || !adj || !work || !misfit) return set_err(SWIT_EINVAL, "all arrays are required");
+    if (kind == SWIT_MISFIT_WAVEFORM)
+        hipLaunchKernelGGL(k_misfit_trace<0>, dim3(ntraces), dim3(kMisfitThreads), 0, stream, nt, dt, obs, syn, adj,
+                           work);
+    else
+        hipLaunchKernelGGL(k_misfit_trace<1>, dim3(ntraces), dim3(kMisfitThreads), 0, stream, nt, dt, obs, syn, adj,
+                           work);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(kMisfitThreads), 0, stream, ntraces, work, misfit);
+    LAUNCH_CHECK();
     return SWIT_OK;
 }

@@ -159,3 +159,88 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+class PlanGroup:
+    """Several batched plans (uneven grids, shot counts, record lengths) run as one job:
+    each forward / adjoint is a single hipGraph launch with every plan's time loop on
+    its own branch.  Results equal those of each plan run alone."""
+
+    def __init__(self, plans):
+        self.plans = list(plans)
+        if not self.plans:
+            raise ValueError("need at least one plan")
+        self._lib = self.plans[0]._lib
+        self._torch = self.plans[0]._torch
+        arr = (ctypes.c_void_p * len(self.plans))(*[p._h.value for p in self.plans])
+        h = ctypes.c_void_p()
+        _native.check(self._lib.swit_group_create(arr, len(self.plans), ctypes.byref(h)))
+        self._h = h
+
+    @staticmethod
+    def _ptrs(tensors):
+        return (ctypes.c_void_p * len(tensors))(*[t.data_ptr() for t in tensors])
+
+    def forward(self, srcs, save_strips=True):
+        """srcs: one (ns_i, nt_i) source array per plan -> list of (ns_i, ngmax_i, nt_i)."""
+        torch = self._torch
+        ins = [p._dev(s, torch.float32, (p.ns, p.nt)) for p, s in zip(self.plans, srcs, strict=True)]
+        outs = [torch.empty((p.ns, p.ngmax, p.nt), dtype=torch.float32, device=p.device) for p in self.plans]
+        _native.check(self._lib.swit_group_forward(self._h, self._ptrs(ins), self._ptrs(outs), int(save_strips),
+                                                   self.plans[0]._stream()))
+        for p, s in zip(self.plans, ins):
+            p._src = s
+        return outs
+
+    def adjoint(self, srcs, adjsrcs):
+        """-> list of (grad, illum_forw, illum_back) per plan, each (nx_i, nz_i) float64."""
+        torch = self._torch
+        ins = [p._dev(s, torch.float32, (p.ns, p.nt)) for p, s in zip(self.plans, srcs, strict=True)]
+        adj = [p._dev(a, torch.float32, (p.ns, p.ngmax, p.nt)) for p, a in zip(self.plans, adjsrcs, strict=True)]
+        outs = [[torch.empty((p.nx, p.nz), dtype=torch.float64, device=p.device) for _ in range(3)]
+                for p in self.plans]
+        _native.check(self._lib.swit_group_adjoint(
+            self._h, self._ptrs(ins), self._ptrs(adj), self._ptrs([o[0] for o in outs]),
+            self._ptrs([o[1] for o in outs]), self._ptrs([o[2] for o in outs]), self.plans[0]._stream()))
+        return [tuple(o) for o in outs]
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.swit_group_destroy(h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_MISFIT_KIND = {"waveform": _native.SWIT_MISFIT_WAVEFORM,
+                "globalcorrelation": _native.SWIT_MISFIT_GLOBAL_CORRELATION}
+
+
+def misfit(kind, obs, syn, dt):
+    """Misfit and adjoint source of device gathers (toolbox/misfit.py rules).
+
+    kind: 'Waveform' or 'Globalcorrelation'; obs, syn: float32 device tensors of one
+    shape (..., nt).  Returns (misfit as a float64 device scalar, adjoint source of
+    syn's shape, per-trace misfits)."""
+    torch = _require_gpu()
+    lib = _native.load()
+    code = _MISFIT_KIND[kind.lower()]
+    if obs.shape != syn.shape:
+        raise ValueError("obs and syn are not consistent.")
+    o = obs.to(torch.float32).contiguous()
+    s = syn.to(torch.float32).contiguous()
+    nt = int(o.shape[-1])
+    ntr = int(o.numel() // nt)
+    adj = torch.empty_like(s)
+    work = torch.empty(ntr, dtype=torch.float64, device=s.device)
+    out = torch.empty(1, dtype=torch.float64, device=s.device)
+    stream = ctypes.c_void_p(torch.cuda.current_stream(s.device).cuda_stream)
+    _native.check(lib.swit_misfit(code, ntr, nt, float(np.float32(dt)), ctypes.c_void_p(o.data_ptr()),
+                                  ctypes.c_void_p(s.data_ptr()), ctypes.c_void_p(adj.data_ptr()),
+                                  ctypes.c_void_p(work.data_ptr()), ctypes.c_void_p(out.data_ptr()), stream))
+    return out[0], adj, work.view(o.shape[:-1])
