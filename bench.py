@@ -81,8 +81,13 @@ def main():
     ap.add_argument("--path", default="batched", choices=["batched", "plain"])
     ap.add_argument("--strips", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch configuration")
     ap.add_argument("--cpu-shots", type=int, default=0, help="CPU baseline sample size (0: auto)")
     ap.add_argument("--plain-shots", type=int, default=2, help="plain-path comparison sample (0: skip)")
+    ap.add_argument("--workload", default="c2", choices=["c2", "uneven"],
+                    help="c2: the 481x141x4000, 49-shot gradient (default); uneven: three grid "
+                         "families (501x171x5000, 481x121x4000, 301x101x3000; 64 shots) as one plan "
+                         "group with the global-correlation misfit on the device")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (5 times): the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -105,6 +110,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "uneven":
+        return bench_uneven(args)
+
     import torch
     from swit_amd import workloads
     from swit_amd.engine import Plan
@@ -122,6 +130,7 @@ def main():
 
     plan = Plan(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, w.src_xz, w.rec_xz, path=args.path, strips=args.strips)
     plan.set_model(vp_true, rho_true)
+    tuned = plan.autotune(steps=50) if args.path == "batched" and not args.no_autotune else None
     obs = plan.forward(src, save_strips=False).clone()
     torch.cuda.synchronize()
 
@@ -206,13 +215,85 @@ def main():
         "data": "synthetic (seeded 12-layer dipping model with two faults, Ricker 5 Hz)",
         "config": {"workload": f"{w.nx}x{w.nz} grid, npml {w.npml}, nt {w.nt}, dt {w.dt}, {w.ns} shots, "
                                f"{w.ng} receivers, L2 gradient", "path": args.path, "strips": args.strips,
-                   "shots_per_step": w.ns},
+                   "shots_per_step": w.ns, "autotuned": tuned},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain_baseline": plain,
     }
     print(json.dumps(line), flush=True)
     plan.close()
+
+
+def bench_uneven(args):
+    """Three grid families in one plan group: each step is one grouped forward with
+    strips, the global-correlation misfit and adjoint source on the device, and one
+    grouped adjoint.  The same work run plan by plan is timed beside it."""
+    import torch
+    from scipy.integrate import cumulative_trapezoid
+    from swit_amd import workloads
+    from swit_amd.engine import Plan, PlanGroup, misfit
+
+    ws = workloads.config4()
+    dev = torch.device("cuda", 0)
+    srcs, plans = [], []
+    for w in ws:
+        wav = workloads.ricker(w.nt, w.dt, w.f0)
+        srcs.append(torch.from_numpy(np.broadcast_to(cumulative_trapezoid(wav, initial=0).astype(np.float32),
+                                                     (w.ns, w.nt)).copy()).to(dev))
+        p = Plan(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, w.src_xz, w.rec_xz, path="batched", strips=args.strips)
+        p.set_model(torch.from_numpy(w.vp_true).to(dev), torch.from_numpy(w.rho(w.vp_true)).to(dev))
+        if not args.no_autotune:
+            p.autotune(steps=30)
+        plans.append(p)
+    group = PlanGroup(plans)
+    obs = [o.clone() for o in group.forward(srcs, save_strips=False)]
+    models = [(torch.from_numpy(w.vp_init).to(dev), torch.from_numpy(w.rho(w.vp_init)).to(dev)) for w in ws]
+    torch.cuda.synchronize()
+
+    def step(grouped):
+        for p, (vp, rho) in zip(plans, models):
+            p.set_model(vp, rho)
+        if grouped:
+            syn = group.forward(srcs, save_strips=True)
+        else:
+            syn = [p.forward(s, save_strips=True) for p, s in zip(plans, srcs)]
+        res = [misfit("Globalcorrelation", o, s_, w.dt) for o, s_, w in zip(obs, syn, ws)]
+        adj = [r[1] for r in res]
+        if grouped:
+            out = group.adjoint(srcs, adj)
+        else:
+            out = [p.adjoint(s, a) for p, s, a in zip(plans, srcs, adj)]
+        return sum(r[0] for r in res), out
+
+    def timed(grouped):
+        for _ in range(args.warmup):
+            step(grouped)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            J, _ = step(grouped)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, float(J)
+
+    el, J = timed(True)
+    el_seq, J_seq = timed(False)
+    nshots = sum(w.ns for w in ws)
+    line = {
+        "metric": "shot-gradients/s, uneven 3-grid batch in one plan group (global-correlation misfit)",
+        "value": round(nshots * args.steps / el, 3), "unit": "shot-gradients/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1000 * el / args.steps, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded layered-plus-fault models 1500-6000 m/s, Ricker 5 Hz)",
+        "config": {"workload": "; ".join(f"{w.nx}x{w.nz} dx {w.dx} nt {w.nt} dt {w.dt}, {w.ns} shots" for w in ws),
+                   "path": "batched group", "strips": args.strips, "shots_per_step": nshots},
+        "misfit": J,
+        "plan_by_plan": {"value": round(nshots * args.steps / el_seq, 3), "ms_per_step": round(1000 * el_seq / args.steps, 2),
+                         "misfit": J_seq, "group_over_plan_by_plan": round(el_seq / el, 3)},
+    }
+    print(json.dumps(line), flush=True)
+    group.close()
+    for p in plans:
+        p.close()
 
 
 if __name__ == "__main__":

@@ -113,7 +113,18 @@ const char *swit_last_error(void);
 #define SWIT_OPT_GRAPH 1        /* 1 (default): replay the time loop as a hipGraph    */
 #define SWIT_OPT_PROFILE 2      /* 1: time each step-kernel time loop with HIP events */
 #define SWIT_OPT_FWD_GROUP 4    /* shots per workgroup of the forward step kernel     */
+#define SWIT_OPT_FWD_COLGROUP 8 /* tile columns per group of the forward work order   */
+#define SWIT_OPT_BWD_COLGROUP 16 /* tile columns per group of the backward work order */
 int swit_plan_set_option(swit_plan *plan, int option, int value);
+
+/* Autotuner (no counterpart in the reference): times `steps` back-to-back launches of
+ * each batched step kernel (best of three bursts) on this plan's grid for every candidate forward
+ * shots-per-workgroup (1..8) x column-group order and every backward column-group
+ * order, and keeps the fastest.  Call between jobs with the model set; saved strips
+ * stay valid (the wavefield buffers it uses are cleared by every job).
+ * choice[3] = {forward shots per workgroup, forward column group, backward column
+ * group}; us[2] = the chosen forward / backward step-kernel time in microseconds. */
+int swit_plan_autotune(swit_plan *plan, int steps, int *choice, double *us, hipStream_t stream);
 
 /* Accumulated duration and launch count of the step kernels timed while
  * SWIT_OPT_PROFILE was on: which = 0 forward step kernel, 1 backward step kernel.

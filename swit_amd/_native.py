@@ -32,6 +32,7 @@ EXPORTS = (
     "swit_last_error",
     "swit_plan_scratch_bytes",
     "swit_plan_strips",
+    "swit_plan_autotune",
     "swit_misfit",
     "swit_group_create",
     "swit_group_forward",
@@ -46,6 +47,8 @@ SWIT_MISFIT_GLOBAL_CORRELATION = 1
 SWIT_OPT_GRAPH = 1
 SWIT_OPT_PROFILE = 2
 SWIT_OPT_FWD_GROUP = 4
+SWIT_OPT_FWD_COLGROUP = 8
+SWIT_OPT_BWD_COLGROUP = 16
 
 
 class SwitGrid(ctypes.Structure):
@@ -107,6 +110,8 @@ def load(path: str | None = None):
     L.swit_plan_destroy.restype = ctypes.c_int
     L.swit_last_error.argtypes = []
     L.swit_last_error.restype = ctypes.c_char_p
+    L.swit_plan_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double), vp]
+    L.swit_plan_autotune.restype = ctypes.c_int
     L.swit_misfit.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, vp, vp, vp, vp, vp, vp]
     L.swit_misfit.restype = ctypes.c_int
     pp = ctypes.POINTER(vp)

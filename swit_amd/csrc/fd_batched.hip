@@ -73,6 +73,7 @@ struct TileMap {
     int th;         // tile height (rows)
     int nzt, nxt;
     int ntiles;     // nzt * nxt
+    int cg;         // tile columns per column group of the work order (see work_item)
 };
 
 struct TileBox {
@@ -96,16 +97,16 @@ __device__ __forceinline__ int xcd_logical(int b, int nb) {
     return k * q + min(k, r) + j;
 }
 
-// Logical work item -> (shot group, tile).  Items run through groups of kColGroup
-// tile columns; inside a group every shot group visits all of the group's tiles
-// before the next one starts.  A group's coefficients (shared by all shots) and the
-// halos its tiles share stay in the XCD's L2 while the shots stream through it.
-constexpr int kColGroup = 4;
+// Logical work item -> (shot group, tile).  Items run through groups of tm.cg tile
+// columns; inside a group every shot group visits all of the group's tiles before the
+// next one starts.  A group's coefficients (shared by all shots) and the halos its
+// tiles share stay in the XCD's L2 while the shots stream through it.
+constexpr int kColGroup = 4;     // default; the autotuner may choose another
 __device__ __forceinline__ void work_item(int w, int nshot, const TileMap &tm, int &s, int &t) {
-    const int per = kColGroup * tm.nzt;              // tiles of a full column group
+    const int per = tm.cg * tm.nzt;                  // tiles of a full column group
     const int gq = w / (per * nshot);
     const int rem = w - gq * per * nshot;
-    const int cols = min(kColGroup, tm.nxt - gq * kColGroup);
+    const int cols = min(tm.cg, tm.nxt - gq * tm.cg);
     const int tiles = cols * tm.nzt;
     s = rem / tiles;
     t = gq * per + rem % tiles;

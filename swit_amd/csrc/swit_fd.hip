@@ -429,6 +429,7 @@ static TileMap make_tilemap(const Dims &d, int ztop, int th) {
     tm.nzt = cdiv(d.nzp - ztop + 1, th);
     tm.nxt = cdiv(d.nxp, kTX);
     tm.ntiles = tm.nzt * tm.nxt;
+    tm.cg = std::min(kColGroup, tm.nxt);
     return tm;
 }
 
@@ -457,6 +458,7 @@ static TileMap backward_tilemap(const Dims &d) {
         tm.nzt = cdiv(d.nzp - 1 - ztop + 1, th);           // cover rows up to nzp-1
         tm.nxt = cdiv(d.nxp, kTX);
         tm.ntiles = tm.nzt * tm.nxt;
+        tm.cg = std::min(kColGroup, tm.nxt);
         if (tm.ntiles < best.ntiles) best = tm;
     }
     return best;
@@ -835,6 +837,116 @@ static int batched_adjoint(swit_plan *pl, const float *src, double *grad, double
 
 // ---- options and measurement -----------------------------------------------------------
 
+// Captured graphs hold the launch arguments: drop them when those change.
+static void drop_graphs(BatchedState &bs) {
+    for (auto &g : bs.gfwd)
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    if (bs.gbwd) { (void)hipGraphExecDestroy(bs.gbwd); bs.gbwd = nullptr; }
+    ++bs.gen;
+}
+
+// Average duration (us) of `steps` back-to-back step-kernel launches (which = 0 forward
+// without strips, 1 backward) on the plan's buffers, after two warm-up launches.
+static int time_steps(swit_plan *pl, int which, int steps, hipStream_t st, double *us) {
+    BatchedState &bs = pl->bat;
+    const Dims &d = pl->d;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    hipError_t e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return set_err(SWIT_EHIP, "hipEventCreate: %s", hipGetErrorString(e));
+    }
+    auto launch = [&](int k) {
+        if (which == 0) {
+            const int it = 2 + k % (d.nt - 1);
+            switch (bs.strip_mode) {
+            case 0: launch_fwd<0>(pl, it, bs.F[k & 1], bs.F[(k + 1) & 1], 0, st); break;
+            case 1: launch_fwd<1>(pl, it, bs.F[k & 1], bs.F[(k + 1) & 1], 0, st); break;
+            default: launch_fwd<2>(pl, it, bs.F[k & 1], bs.F[(k + 1) & 1], 0, st); break;
+            }
+        } else {
+            const int it = d.nt - k % (d.nt - 1);
+            const float *ri = bs.R[k & 1], *bi = bs.Bk[k & 1];
+            float *ro = bs.R[(k + 1) & 1], *bo = bs.Bk[(k + 1) & 1];
+            switch (bs.strip_mode) {
+            case 0: launch_bwd<0>(pl, it, ri, ro, bi, bo, st); break;
+            case 1: launch_bwd<1>(pl, it, ri, ro, bi, bo, st); break;
+            default: launch_bwd<2>(pl, it, ri, ro, bi, bo, st); break;
+            }
+        }
+    };
+    int rc = SWIT_OK;
+    for (int k = 0; k < 2; ++k) launch(k);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int k = 0; k < steps && e == hipSuccess; ++k) launch(k);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) rc = set_err(SWIT_EHIP, "autotune timing: %s", hipGetErrorString(e));
+    *us = 1000.0 * ms / steps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+extern "C" int swit_plan_autotune(swit_plan *pl, int steps, int *choice, double *us, hipStream_t stream) {
+    g_err[0] = 0;
+    if (!pl || !choice || !us) return set_err(SWIT_EINVAL, "plan, choice and us are required");
+    if (!(pl->flags & SWIT_PATH_BATCHED)) return set_err(SWIT_EINVAL, "the autotuner tunes the batched path");
+    if (!pl->model_set) return set_err(SWIT_ESTATE, "swit_plan_set_model must be called first");
+    if (steps < 1 || steps > 1000) return set_err(SWIT_EINVAL, "steps must be in [1, 1000]");
+    BatchedState &bs = pl->bat;
+    HIP_TRY(hipStreamSynchronize(stream));
+    // the wavefield buffers are scratch here; every job clears what it reads first
+    static const int kGroups[] = {1, 2, 3, 4, 5, 6, 8};
+    static const int kCols[] = {1, 2, 4, 8, 16};
+    constexpr int kRepeats = 3;      // each candidate keeps its best of three bursts
+    double best_f = 1e30, best_b = 1e30;
+    int bg = bs.Gf, bcf = bs.fwd.cg, bcb = bs.bwd.cg;
+    for (int G : kGroups) {
+        if (G > pl->ns) continue;
+        for (int cg : kCols) {
+            if (cg > bs.fwd.nxt) continue;
+            bs.Gf = G;
+            bs.ngroups_f = cdiv(pl->ns, G);
+            bs.fwd.cg = cg;
+            double t = 1e30;
+            for (int r = 0; r < kRepeats; ++r) {
+                double tr;
+                TRY(time_steps(pl, 0, steps, stream, &tr));
+                t = std::min(t, tr);
+            }
+            if (t < best_f) { best_f = t; bg = G; bcf = cg; }
+        }
+    }
+    for (int cg : kCols) {
+        if (cg > bs.bwd.nxt) continue;
+        bs.bwd.cg = cg;
+        double t = 1e30;
+        for (int r = 0; r < kRepeats; ++r) {
+            double tr;
+            TRY(time_steps(pl, 1, steps, stream, &tr));
+            t = std::min(t, tr);
+        }
+        if (t < best_b) { best_b = t; bcb = cg; }
+    }
+    bs.Gf = bg;
+    bs.ngroups_f = cdiv(pl->ns, bg);
+    bs.fwd.cg = bcf;
+    bs.bwd.cg = bcb;
+    drop_graphs(bs);
+    choice[0] = bg;
+    choice[1] = bcf;
+    choice[2] = bcb;
+    us[0] = best_f;
+    us[1] = best_b;
+    return SWIT_OK;
+}
+
 extern "C" int swit_plan_set_option(swit_plan *pl, int option, int value) {
     g_err[0] = 0;
     if (!pl) return set_err(SWIT_EINVAL, "plan is NULL");
@@ -855,10 +967,16 @@ extern "C" int swit_plan_set_option(swit_plan *pl, int option, int value) {
         if (value < 1 || value > pl->ns) return set_err(SWIT_EINVAL, "group size %d outside [1, ns]", value);
         bs.Gf = value;
         bs.ngroups_f = cdiv(pl->ns, value);
-        ++bs.gen;
-        for (auto &gx : bs.gfwd)
-            if (gx) { (void)hipGraphExecDestroy(gx); gx = nullptr; }
+        drop_graphs(bs);
         return SWIT_OK;
+    case SWIT_OPT_FWD_COLGROUP:
+    case SWIT_OPT_BWD_COLGROUP: {
+        TileMap &tm = option == SWIT_OPT_FWD_COLGROUP ? bs.fwd : bs.bwd;
+        if (value < 1 || value > tm.nxt) return set_err(SWIT_EINVAL, "column group %d outside [1, %d]", value, tm.nxt);
+        tm.cg = value;
+        drop_graphs(bs);
+        return SWIT_OK;
+    }
     default:
         return set_err(SWIT_EINVAL, "unknown option %d", option);
     }

@@ -93,8 +93,19 @@ class Plan:
     def set_option(self, option, value):
         """option: 'graph' | 'profile' | 'fwd_group' (batched path)."""
         code = {"graph": _native.SWIT_OPT_GRAPH, "profile": _native.SWIT_OPT_PROFILE,
-                "fwd_group": _native.SWIT_OPT_FWD_GROUP}[option]
+                "fwd_group": _native.SWIT_OPT_FWD_GROUP, "fwd_colgroup": _native.SWIT_OPT_FWD_COLGROUP,
+                "bwd_colgroup": _native.SWIT_OPT_BWD_COLGROUP}[option]
         _native.check(self._lib.swit_plan_set_option(self._h, code, int(value)))
+
+    def autotune(self, steps=20):
+        """Time the batched step kernels for every candidate forward shots-per-workgroup
+        and column-group order (backward: column-group order) and keep the fastest.
+        Returns {'fwd_group', 'fwd_colgroup', 'bwd_colgroup', 'fwd_us', 'bwd_us'}."""
+        choice = (ctypes.c_int * 3)()
+        us = (ctypes.c_double * 2)()
+        _native.check(self._lib.swit_plan_autotune(self._h, int(steps), choice, us, self._stream()))
+        return {"fwd_group": choice[0], "fwd_colgroup": choice[1], "bwd_colgroup": choice[2],
+                "fwd_us": round(us[0], 2), "bwd_us": round(us[1], 2)}
 
     def kernel_time(self, which):
         """(total_ms, launches) of the forward (which='forward') or backward step kernel

@@ -129,3 +129,30 @@ def test_batched_tilings(nx, nz, npml, nt, ns):
         assert rel_l2(g, a["grad"]) <= 1e-12, (graph, rel_l2(g, a["grad"]))
         assert rel_l2(fw, a["illum_forw"]) <= 1e-12
         assert rel_l2(bk, a["illum_back"]) <= 1e-12
+
+
+@pytest.mark.gpu
+def test_autotune_keeps_results_and_saved_strips():
+    """The autotuner only changes the work order and shots per workgroup: results are
+    bit-identical before and after, and strips saved before tuning stay usable."""
+    import torch
+    w, src, adj, f, a = _case(seed=15, ns=5, nt=160)
+    p = _plan(w)
+    seis0, g0 = _run(p, src, adj)
+    p.forward(torch.from_numpy(src), save_strips=True)
+    ch = p.autotune(steps=5)
+    print("autotune", ch)
+    assert 1 <= ch["fwd_group"] <= w.ns and ch["fwd_colgroup"] >= 1 and ch["bwd_colgroup"] >= 1
+    assert ch["fwd_us"] > 0 and ch["bwd_us"] > 0
+    g_after = [t.cpu().numpy() for t in p.adjoint(torch.from_numpy(src), torch.from_numpy(adj))]
+    seis1, g1 = _run(p, src, adj)
+    for opt, v in (("fwd_colgroup", 1), ("bwd_colgroup", 3), ("fwd_group", 2)):
+        p.set_option(opt, v)
+    seis2, g2 = _run(p, src, adj)
+    p.close()
+    np.testing.assert_array_equal(seis1, seis0)
+    np.testing.assert_array_equal(seis2, seis0)
+    for x, y, z, q in zip(g_after, g1, g2, g0):
+        np.testing.assert_array_equal(x, q)
+        np.testing.assert_array_equal(y, q)
+        np.testing.assert_array_equal(z, q)
