@@ -39,6 +39,8 @@ extern "C" {
 #define SWIT_PATH_BATCHED 0x1   /* all shots per launch, fused steps, graph-captured */
 #define SWIT_STRIPS_F16 0x10    /* boundary strips stored as FP16 with per-step scale */
 #define SWIT_STRIPS_BF16 0x20   /* boundary strips stored as BF16 with per-step scale */
+#define SWIT_PLAN_EXACT 0x40   /* also allocate the exact adjoint: divergence history of
+                                   ns * nt * (nx+2npml) * ld floats */
 
 /* Grid and time axis: the reference's param type (fd2dmpi/datatype.f90:17-22) as
  * read from the parfile (io.f90:72-96).  dx and dt are the float32 values the
@@ -137,6 +139,18 @@ int swit_plan_kernel_time(swit_plan *plan, int which, double *total_ms, int64_t 
  * reference's boundary.bin per shot (fdcore.f90:130-159,166-176).
  * SWIT_ESTATE if no strips are saved. */
 int swit_plan_strips(swit_plan *plan, float *out, hipStream_t stream);
+
+/* ---- exact adjoint (no counterpart in the reference) ----------------------------------
+ * The transpose of the forward scheme itself (the reference's adjoint, swit_adjoint,
+ * uses a different discretisation and is not the exact gradient).  For a misfit J of
+ * the seismograms, swit_adjoint_exact turns dJ/dseis into dJ/dvp through
+ * K = den c^2 dt/dx and the PML padding; density and the damping profile are held
+ * fixed.  Needs a batched plan created with SWIT_PLAN_EXACT.
+ *   swit_forward_exact: as swit_forward (same seismograms, bit for bit), also storing
+ *     the divergence term of every pressure update (no strips are saved);
+ *   swit_adjoint_exact: gseis [ns][ngmax][nt] = dJ/dseis; grad [nx][nz] float64. */
+int swit_forward_exact(swit_plan *plan, const float *src, float *seis, hipStream_t stream);
+int swit_adjoint_exact(swit_plan *plan, const float *gseis, double *grad, hipStream_t stream);
 
 /* ---- misfits and adjoint sources (toolbox/misfit.py) -------------------------------
  * obs, syn, adj: [ntraces][nt] device arrays (a plan's [ns][ngmax][nt] seismograms

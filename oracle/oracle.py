@@ -84,12 +84,16 @@ def _dt(precision):
 class Model:
     """Padded model + damping, as set up by solver.f90:53-76."""
 
-    def __init__(self, vp, rho, dx, npml, precision="f32", damp_override=None):
+    def __init__(self, vp, rho, dx, npml, precision="f32", damp_override=None, exact_input=False):
+        """exact_input: with precision 'f64', keep vp / rho as given instead of rounding
+        them through float32 (the reference reads float32 files); used to difference
+        models that differ by less than a float32 step."""
         L = lib()
         self.precision = precision
         T = _dt(precision)
-        vp = np.ascontiguousarray(np.asarray(vp, dtype=np.float32).astype(T))
-        rho = np.ascontiguousarray(np.asarray(rho, dtype=np.float32).astype(T))
+        keep = exact_input and precision == "f64"
+        vp = np.ascontiguousarray(np.asarray(vp, dtype=np.float64 if keep else np.float32).astype(T))
+        rho = np.ascontiguousarray(np.asarray(rho, dtype=np.float64 if keep else np.float32).astype(T))
         self.nx, self.nz = vp.shape
         self.npml = int(npml)
         self.dx = float(np.float32(dx))

@@ -21,6 +21,7 @@ SWIT_PATH_PLAIN = 0x0
 SWIT_PATH_BATCHED = 0x1
 SWIT_STRIPS_F16 = 0x10
 SWIT_STRIPS_BF16 = 0x20
+SWIT_PLAN_EXACT = 0x40
 
 # Every entry point include/swit_fd.h declares.
 EXPORTS = (
@@ -33,6 +34,8 @@ EXPORTS = (
     "swit_plan_scratch_bytes",
     "swit_plan_strips",
     "swit_plan_autotune",
+    "swit_forward_exact",
+    "swit_adjoint_exact",
     "swit_misfit",
     "swit_group_create",
     "swit_group_forward",
@@ -110,6 +113,10 @@ def load(path: str | None = None):
     L.swit_plan_destroy.restype = ctypes.c_int
     L.swit_last_error.argtypes = []
     L.swit_last_error.restype = ctypes.c_char_p
+    L.swit_forward_exact.argtypes = [vp, vp, vp, vp]
+    L.swit_forward_exact.restype = ctypes.c_int
+    L.swit_adjoint_exact.argtypes = [vp, vp, vp, vp]
+    L.swit_adjoint_exact.restype = ctypes.c_int
     L.swit_plan_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double), vp]
     L.swit_plan_autotune.restype = ctypes.c_int
     L.swit_misfit.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, vp, vp, vp, vp, vp, vp]

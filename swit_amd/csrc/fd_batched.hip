@@ -268,7 +268,8 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
     const ShotGeom *__restrict__ geo, const float *__restrict__ src, const float *__restrict__ In,
     float *__restrict__ Out, const int *__restrict__ rec_start, const RecRef *__restrict__ rec_list,
     float *__restrict__ seis_t, const int *__restrict__ strip_start, const StripRef *__restrict__ strip_list,
-    void *__restrict__ strips, float *__restrict__ stage, uint32_t *__restrict__ smax) {
+    void *__restrict__ strips, float *__restrict__ stage, uint32_t *__restrict__ smax,
+    float *__restrict__ dsave) {
     using namespace fk;
     __shared__ float sP[kNR];
     __shared__ float sU[kNR];
@@ -360,8 +361,12 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
             if (qgrid >> c & 1u) {
                 v = sP[l];
                 if (qupd >> c & 1u) {
-                    v = p_update_f(v, cp[c].x, cp[c].y, sU[l], sU[l - kRZ], sU[l + kRZ], sU[l - 2 * kRZ],
-                                   sW[l], sW[l - 1], sW[l + 1], sW[l - 2]);
+                    const float dv = div_term(sU[l], sU[l - kRZ], sU[l + kRZ], sU[l - 2 * kRZ], sW[l], sW[l - 1],
+                                              sW[l + 1], sW[l - 2]);
+                    v = cp[c].y * v - cp[c].x * dv;
+                    // divergence history for the exact adjoint: tile cells, slot it-1
+                    if (dsave && z >= tb.z0 && z <= tb.z1 && x >= tb.x0 && x <= tb.x1)
+                        dsave[((size_t)s * d.nt + (it - 1)) * np + (unsigned)((x - 1) * d.ld + (z - 1))] = dv;
                     if (z == g.isz && x == g.isx) v = v + srcv;
                 }
             }
@@ -782,6 +787,11 @@ struct BatchedState {
     float *end = nullptr;        // saved final forward state [ns][3][nx][nz]
     float *src = nullptr;        // plan-owned copy of the source traces
     float *acc = nullptr;        // [ns][3][nx][nz] per-shot FP32 accumulators
+    float *dhist = nullptr;      // [ns][nt][nxp][ld] divergence history (exact adjoint)
+    float *lam = nullptr;        // [ns][3][nxp][ld] adjoint variables of the exact adjoint
+    float *qbuf = nullptr;       // [ns][2][nxp][ld] exact-adjoint temporaries
+    float *gk = nullptr;         // [ns][nxp][ld] per-shot gradient with respect to K
+    bool exact_valid = false;    // dhist holds the last forward_exact of the current model
     int strip_mode = 0;          // 0 FP32, 1 FP16, 2 BF16
     uint16_t *strips16 = nullptr;   // [ns][nt][S] 16-bit strips
     uint32_t *smax = nullptr;    // [ns][nt] per-row max |v| (float bits)

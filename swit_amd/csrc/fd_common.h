@@ -73,10 +73,15 @@ __device__ __forceinline__ float v_update(float v, float alpha, float kappa,
 
 // The same updates with the damping factor (1 - kappa) precomputed: the reference
 // rounds kappa, then 1 - kappa, then multiplies, so storing 1 - kappa keeps every bit.
+// Divergence term of the pressure update: c1 (u0-um1+w0-wm1) + c2 (up1-um2+wp1-wm2).
+__device__ __forceinline__ float div_term(float u0, float um1, float up1, float um2,
+                                          float w0, float wm1, float wp1, float wm2) {
+    return kC1 * (u0 - um1 + w0 - wm1) + kC2 * (up1 - um2 + wp1 - wm2);
+}
 __device__ __forceinline__ float p_update_f(float p, float alpha, float fac,
                                             float u0, float um1, float up1, float um2,
                                             float w0, float wm1, float wp1, float wm2) {
-    return fac * p - alpha * (kC1 * (u0 - um1 + w0 - wm1) + kC2 * (up1 - um2 + wp1 - wm2));
+    return fac * p - alpha * div_term(u0, um1, up1, um2, w0, wm1, wp1, wm2);
 }
 __device__ __forceinline__ float v_update_f(float v, float alpha, float fac,
                                             float q0, float q1, float q2, float qm1) {

@@ -18,6 +18,7 @@
 #include "fd_plain.hip"
 #include "fd_batched.hip"
 #include "fd_misfit.hip"
+#include "fd_exact.hip"
 
 using namespace swit;
 
@@ -63,7 +64,7 @@ struct swit_plan {
     uint32_t *strip_map = nullptr;
     // model
     float *K = nullptr, *B = nullptr, *damp = nullptr, *prof = nullptr, *vmin = nullptr;
-    float *c_int = nullptr;
+    float *c_int = nullptr, *rho_int = nullptr;   // interior vp and density of the model
     // wavefields
     float *P = nullptr, *U = nullptr, *W = nullptr;
     float *Pb[2] = {nullptr, nullptr}, *Ub = nullptr, *Wb = nullptr;
@@ -119,6 +120,7 @@ static int plan_build(swit_plan *pl, const swit_grid *g, const swit_shots *sh, i
     const bool s16 = (flags & (SWIT_STRIPS_F16 | SWIT_STRIPS_BF16)) != 0;
     const bool batched = (flags & SWIT_PATH_BATCHED) != 0;
     if (s16 && !batched) return set_err(SWIT_EINVAL, "16-bit strips need the batched path");
+    if ((flags & SWIT_PLAN_EXACT) && !batched) return set_err(SWIT_EINVAL, "the exact adjoint needs the batched path");
 
     Dims &d = pl->d;
     d.nx = g->nx;
@@ -205,6 +207,7 @@ static int plan_build(swit_plan *pl, const swit_grid *g, const swit_shots *sh, i
     TRY(pl->alloc(&pl->prof, 1024));
     TRY(pl->alloc(&pl->vmin, 64));
     TRY(pl->alloc(&pl->c_int, ni));
+    TRY(pl->alloc(&pl->rho_int, ni));
     if (!batched) {
         // per-shot wavefields of the plain path
         TRY(pl->alloc(&pl->P, np * ns));
@@ -300,8 +303,10 @@ extern "C" int swit_plan_set_model(swit_plan *pl, const float *vp, const float *
     hipLaunchKernelGGL(k_setup_damp, grd, blk, 0, stream, d, pl->prof, pl->damp);
     LAUNCH_CHECK();
     HIP_TRY(hipMemcpyAsync(pl->c_int, vp, sizeof(float) * d.nx * d.nz, hipMemcpyDeviceToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(pl->rho_int, rho, sizeof(float) * d.nx * d.nz, hipMemcpyDeviceToDevice, stream));
     pl->model_set = true;
     pl->strips_valid = false;
+    pl->bat.exact_valid = false;
     if (pl->flags & SWIT_PATH_BATCHED) TRY(batched_set_model(pl, stream));
     return SWIT_OK;
 }
@@ -576,6 +581,12 @@ static int batched_init(swit_plan *pl) {
     TRY(pl->alloc(&bs.src, (size_t)ns * d.nt));
     TRY(pl->alloc(&bs.acc, (size_t)ns * 3 * ni));
     TRY(pl->alloc(&bs.out, 3 * ni));
+    if (pl->flags & SWIT_PLAN_EXACT) {
+        TRY(pl->alloc(&bs.dhist, (size_t)ns * d.nt * np));
+        TRY(pl->alloc(&bs.lam, (size_t)ns * 3 * np));
+        TRY(pl->alloc(&bs.qbuf, (size_t)ns * 2 * np));
+        TRY(pl->alloc(&bs.gk, (size_t)ns * np));
+    }
     bs.strip_mode = (pl->flags & SWIT_STRIPS_F16) ? 1 : (pl->flags & SWIT_STRIPS_BF16) ? 2 : 0;
     if (bs.strip_mode) {
         TRY(pl->alloc(&bs.strips16, (size_t)ns * d.nt * d.S));
@@ -673,13 +684,14 @@ static int forward_prologue(swit_plan *pl, int save, hipStream_t st) {
 }
 
 template <int SM>
-static void launch_fwd(swit_plan *pl, int it, const float *in, float *out, int save, hipStream_t st) {
+static void launch_fwd(swit_plan *pl, int it, const float *in, float *out, int save, hipStream_t st,
+                       float *dsave = nullptr) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
     void *strips = !save ? nullptr : SM == 0 ? static_cast<void *>(pl->strips) : static_cast<void *>(bs.strips16);
     hipLaunchKernelGGL(k_fwd_fused<SM>, dim3(bs.ngroups_f * bs.fwd.ntiles), dim3(kThreads), 0, st, d, it, bs.fwd,
                        pl->ns, bs.Gf, bs.CP, bs.CV, pl->geo, bs.src, in, out, bs.rec_start, bs.rec_list,
-                       pl->seis_t, bs.strip_start, bs.strip_list, strips, bs.stage, bs.smax);
+                       pl->seis_t, bs.strip_start, bs.strip_list, strips, bs.stage, bs.smax, dsave);
 }
 
 template <int SM>
@@ -1201,6 +1213,64 @@ extern "C" int swit_misfit(int kind, int ntraces, int nt, float dt, const float 
                            work);
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(kMisfitThreads), 0, stream, ntraces, work, misfit);
+    LAUNCH_CHECK();
+    return SWIT_OK;
+}
+
+// ---- exact adjoint ---------------------------------------------------------------------------
+
+extern "C" int swit_forward_exact(swit_plan *pl, const float *src, float *seis, hipStream_t stream) {
+    g_err[0] = 0;
+    if (!pl || !src || !seis) return set_err(SWIT_EINVAL, "plan, src and seis are required");
+    if (!(pl->flags & SWIT_PLAN_EXACT)) return set_err(SWIT_EINVAL, "plan was not created with SWIT_PLAN_EXACT");
+    if (!pl->model_set) return set_err(SWIT_ESTATE, "swit_plan_set_model must be called first");
+    BatchedState &bs = pl->bat;
+    const Dims &d = pl->d;
+    bs.exact_valid = false;
+    HIP_TRY(hipMemcpyAsync(bs.src, src, sizeof(float) * pl->ns * d.nt, hipMemcpyDeviceToDevice, stream));
+    TRY(forward_prologue(pl, 0, stream));
+    for (int it = 2; it <= d.nt; ++it) {
+        const float *in = bs.F[it % 2];
+        float *out = bs.F[(it + 1) % 2];
+        switch (bs.strip_mode) {
+        case 0: launch_fwd<0>(pl, it, in, out, 0, stream, bs.dhist); break;
+        case 1: launch_fwd<1>(pl, it, in, out, 0, stream, bs.dhist); break;
+        default: launch_fwd<2>(pl, it, in, out, 0, stream, bs.dhist); break;
+        }
+        LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_transpose, dim3(cdiv(d.ngmax, 32), cdiv(d.nt, 32), pl->ns), dim3(32, 8), 0, stream, d.nt,
+                       d.ngmax, pl->seis_t, seis);
+    LAUNCH_CHECK();
+    bs.exact_valid = true;
+    return SWIT_OK;
+}
+
+extern "C" int swit_adjoint_exact(swit_plan *pl, const float *gseis, double *grad, hipStream_t stream) {
+    g_err[0] = 0;
+    if (!pl || !gseis || !grad) return set_err(SWIT_EINVAL, "plan, gseis and grad are required");
+    if (!(pl->flags & SWIT_PLAN_EXACT)) return set_err(SWIT_EINVAL, "plan was not created with SWIT_PLAN_EXACT");
+    BatchedState &bs = pl->bat;
+    if (!bs.exact_valid) return set_err(SWIT_ESTATE, "no divergence history: run swit_forward_exact first");
+    const Dims &d = pl->d;
+    const int ns = pl->ns;
+    const size_t np = (size_t)d.nxp * d.ld;
+    TRY(fill(bs.lam, 3 * np * ns, stream));
+    TRY(fill(bs.gk, np * ns, stream));
+    const dim3 blk(64, 4), grd(cdiv(d.nzp, 64), cdiv(d.nxp, 4), ns);
+    const dim3 gcol(cdiv(d.nxp, 256), ns), ginj(cdiv(d.ngmax, 256), ns);
+    for (int it = d.nt; it >= 2; --it) {
+        hipLaunchKernelGGL(ex::k_ex_inject, ginj, dim3(256), 0, stream, d, it, pl->geo, pl->rec_fwd, gseis, bs.lam);
+        if (d.free_surface == 1) hipLaunchKernelGGL(ex::k_ex_mirror, gcol, dim3(256), 0, stream, d, 2, bs.lam);
+        hipLaunchKernelGGL(ex::k_ex_vel, grd, blk, 0, stream, d, bs.KB, bs.CV, bs.lam, bs.qbuf);
+        hipLaunchKernelGGL(ex::k_ex_p, grd, blk, 0, stream, d, bs.qbuf, bs.lam);
+        if (d.free_surface == 1) hipLaunchKernelGGL(ex::k_ex_mirror, gcol, dim3(256), 0, stream, d, 0, bs.lam);
+        hipLaunchKernelGGL(ex::k_ex_grad, grd, blk, 0, stream, d, it, bs.KB, bs.CP, bs.dhist, bs.lam, bs.qbuf, bs.gk);
+        hipLaunchKernelGGL(ex::k_ex_vel2, grd, blk, 0, stream, d, bs.qbuf, bs.lam);
+        LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(ex::k_ex_fold, dim3(cdiv(d.nz, 64), cdiv(d.nx, 4)), blk, 0, stream, d, ns, pl->c_int,
+                       pl->rho_int, bs.gk, grad);
     LAUNCH_CHECK();
     return SWIT_OK;
 }

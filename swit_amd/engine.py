@@ -46,7 +46,7 @@ class Plan:
     """
 
     def __init__(self, nx, nz, dx, dt, nt, npml, src_xz, rec_xz, free_surface=1,
-                 path="batched", strips="f32"):
+                 path="batched", strips="f32", exact=False):
         torch = _require_gpu()
         self._torch = torch
         self._lib = _native.load()
@@ -59,6 +59,8 @@ class Plan:
         self.dx, self.dt = float(np.float32(dx)), float(np.float32(dt))
         flags = {"plain": _native.SWIT_PATH_PLAIN, "batched": _native.SWIT_PATH_BATCHED}[path]
         flags |= {"f32": 0, "f16": _native.SWIT_STRIPS_F16, "bf16": _native.SWIT_STRIPS_BF16}[strips]
+        if exact:
+            flags |= _native.SWIT_PLAN_EXACT
         self.path, self.strips = path, strips
         g = _native.SwitGrid(self.nx, self.nz, self.npml, self.nt, int(free_surface),
                              self.dx, self.dt)
@@ -151,6 +153,28 @@ class Plan:
                                              *[ctypes.c_void_p(o.data_ptr()) for o in outs],
                                              self._stream()))
         return tuple(outs)
+
+    def forward_exact(self, src, out=None):
+        """Forward (same seismograms as forward()) that also stores the divergence
+        history for adjoint_exact().  Needs a plan created with exact=True."""
+        torch = self._torch
+        src = self._dev(src, torch.float32, (self.ns, self.nt))
+        if out is None:
+            out = torch.empty((self.ns, self.ngmax, self.nt), dtype=torch.float32, device=self.device)
+        _native.check(self._lib.swit_forward_exact(self._h, ctypes.c_void_p(src.data_ptr()),
+                                                   ctypes.c_void_p(out.data_ptr()), self._stream()))
+        self._src = src
+        return out
+
+    def adjoint_exact(self, gseis):
+        """dJ/dseis (ns, ngmax, nt) -> dJ/dvp (nx, nz) float64: the exact discrete adjoint
+        of forward_exact's scheme (density and damping held fixed)."""
+        torch = self._torch
+        g = self._dev(gseis, torch.float32, (self.ns, self.ngmax, self.nt))
+        out = torch.empty((self.nx, self.nz), dtype=torch.float64, device=self.device)
+        _native.check(self._lib.swit_adjoint_exact(self._h, ctypes.c_void_p(g.data_ptr()),
+                                                   ctypes.c_void_p(out.data_ptr()), self._stream()))
+        return out
 
     def saved_strips(self):
         """Saved boundary strips decoded to FP32: (ns, nt, 9nx+6nz) on the device."""

@@ -40,3 +40,74 @@ def quantize_strips(boundary, mode):
     else:
         raise ValueError(mode)
     return (q * scale).astype(np.float32)
+
+
+def torch_forward(vp, rho, dx, dt, npml, src_xz, rec_xz, src, damp, free_surface=1):
+    """Vectorised FP64 torch restatement of the forward scheme (fdcore.f90:75-128) for
+    checking gradients by autograd: vp (nx, nz) torch tensor (may require grad), rho
+    (nx, nz) and damp (nxp, nzp) numpy, src (ns, nt), src_xz (ns, 2), rec_xz (ns, ng, 2).
+    Returns seismograms (ns, ng, nt) with sample 0 zero."""
+    import torch
+    nx, nz = vp.shape
+    nxp, nzp, fs = nx + 2 * npml, nz + 2 * npml, npml + 1
+    c1, c2 = 9.0 / 8.0, -1.0 / 24.0
+    dt = float(np.float32(dt))
+    dx = float(np.float32(dx))
+
+    def pad(a):
+        return torch.nn.functional.pad(a.T[None, None], (npml,) * 4, mode="replicate")[0, 0]
+
+    C = pad(vp)
+    D = pad(torch.as_tensor(np.asarray(rho, np.float64)))
+    K, B = D * C * C * dt / dx, dt / (D * dx)
+    dmp = torch.as_tensor(np.asarray(damp, np.float64).T)             # (nzp, nxp)
+    fp = 1 - dmp * dt
+    fu = torch.ones_like(dmp)
+    fu[:, :-1] = 1 - 0.5 * (dmp[:, :-1] + dmp[:, 1:]) * dt
+    fw = torch.ones_like(dmp)
+    fw[:-1, :] = 1 - 0.5 * (dmp[:-1, :] + dmp[1:, :]) * dt
+    f32 = np.float32
+    out = []
+    for s in range(src.shape[0]):
+        isx = npml + int(f32(src_xz[s][0]) / f32(dx)) + 1
+        isz = npml + int(f32(src_xz[s][1]) / f32(dx)) + 1
+        if isz == fs:
+            isz += 1
+        rz, rx = [], []
+        for xg, zg in np.asarray(rec_xz[s], np.float32):
+            igx = npml + int(f32(xg) / f32(dx)) + 1
+            igz = npml + int(f32(zg) / f32(dx)) + 1
+            rz.append((fs + 1 if igz == fs else igz) - 1)
+            rx.append(igx - 1)
+        p = torch.zeros(nzp, nxp, dtype=torch.float64)
+        u = torch.zeros_like(p)
+        w = torch.zeros_like(p)
+        rec = [torch.zeros(len(rz), dtype=torch.float64)]
+        # 0-based slices of the update regions
+        pz, px = slice(fs, nzp - 2), slice(2, nxp - 2)
+        uz, ux = slice(fs - 1, nzp - 1), slice(1, nxp - 2)
+        wz, wx = slice(fs - 1, nzp - 2), slice(1, nxp - 1)
+
+        def sh(a, dz, dxx, zs, xs):
+            return a[zs.start + dz:zs.stop + dz, xs.start + dxx:xs.stop + dxx]
+        for it in range(2, src.shape[1] + 1):
+            div = (c1 * (sh(u, 0, 0, pz, px) - sh(u, 0, -1, pz, px) + sh(w, 0, 0, pz, px) - sh(w, -1, 0, pz, px))
+                   + c2 * (sh(u, 0, 1, pz, px) - sh(u, 0, -2, pz, px) + sh(w, 1, 0, pz, px) - sh(w, -2, 0, pz, px)))
+            pn = p.clone()
+            pn[pz, px] = fp[pz, px] * p[pz, px] - K[pz, px] * div
+            pn[isz - 1, isx - 1] = pn[isz - 1, isx - 1] + dt * float(src[s][it - 1])
+            if free_surface == 1:
+                pn[fs - 2, :] = -pn[fs, :]
+            p = pn
+            un = u.clone()
+            un[uz, ux] = fu[uz, ux] * u[uz, ux] - B[uz, ux] * (
+                c1 * (sh(p, 0, 1, uz, ux) - sh(p, 0, 0, uz, ux)) + c2 * (sh(p, 0, 2, uz, ux) - sh(p, 0, -1, uz, ux)))
+            wn = w.clone()
+            wn[wz, wx] = fw[wz, wx] * w[wz, wx] - B[wz, wx] * (
+                c1 * (sh(p, 1, 0, wz, wx) - sh(p, 0, 0, wz, wx)) + c2 * (sh(p, 2, 0, wz, wx) - sh(p, -1, 0, wz, wx)))
+            if free_surface == 1:
+                wn[fs - 2, :] = wn[fs - 1, :]
+            u, w = un, wn
+            rec.append(p[rz, rx])
+        out.append(torch.stack(rec, 1))
+    return torch.stack(out, 0)
