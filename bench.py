@@ -54,6 +54,40 @@ def load_traffic(kernel):
     return t.get("bytes_per_launch")
 
 
+def autograd_leg(w, src, obs, vp_init, rho_init, args):
+    """The same gradient through torch.autograd with the exact adjoint: one step is
+    seismograms(vp) (forward with the divergence history), 1/2 |d - d_obs|^2 and
+    backward (exact adjoint of all shots).  Needs ns * nt * (nx+2npml) * ld * 4 bytes
+    for the history; skipped when the device cannot hold it."""
+    import torch
+    from swit_amd.autograd import l2_misfit, seismograms
+    from swit_amd.engine import Plan
+    need = w.ns * w.nt * (w.nx + 2 * w.npml) * ((w.nz + 2 * w.npml + 31) // 32 * 32) * 4
+    free, _ = torch.cuda.mem_get_info()
+    if need > 0.8 * free:
+        return {"skipped": f"divergence history needs {need / 2**30:.1f} GiB, {free / 2**30:.1f} GiB free"}
+    pe = Plan(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, w.src_xz, w.rec_xz, path="batched", exact=True)
+
+    def step():
+        vp = vp_init.clone().requires_grad_(True)
+        loss = l2_misfit(seismograms(vp, pe, src, rho_init), obs)
+        loss.backward()
+        return vp.grad
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g = step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    pe.close()
+    return {"value": round(w.ns * args.steps / el, 3), "unit": "shot-gradients/s",
+            "ms_per_step": round(1000 * el / args.steps, 2), "history_gib": round(need / 2**30, 1),
+            "finite": bool(torch.isfinite(g).all())}
+
+
 def cpu_baseline(w, src, obs_syn_adj, nshots):
     """FP32 oracle (C restatement of the reference scheme) on host cores: nshots
     shot-gradients of the same workload, one thread per shot."""
@@ -82,6 +116,8 @@ def main():
     ap.add_argument("--strips", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch configuration")
+    ap.add_argument("--no-autograd", action="store_true",
+                    help="skip the exact-adjoint (torch.autograd) gradient leg")
     ap.add_argument("--cpu-shots", type=int, default=0, help="CPU baseline sample size (0: auto)")
     ap.add_argument("--plain-shots", type=int, default=2, help="plain-path comparison sample (0: skip)")
     ap.add_argument("--workload", default="c2", choices=["c2", "uneven"],
@@ -177,6 +213,11 @@ def main():
                                               / (kern["forward" if dom == "backward" else "backward"][0] * 1e-3)
                                               / 1e9, 1)}}
 
+    autograd = None
+    if args.path == "batched" and not args.no_autograd:
+        plan.close()
+        autograd = autograd_leg(w, src, obs, vp_init, rho_init, args)
+
     plain = None
     if args.plain_shots > 0 and args.path == "batched":
         k = min(args.plain_shots, w.ns)
@@ -219,7 +260,10 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain_baseline": plain,
+        "autograd": autograd,
     }
+    if plain and autograd and "value" in autograd:
+        autograd["over_plain"] = round(autograd["value"] / plain["value"], 2)
     print(json.dumps(line), flush=True)
     plan.close()
 

@@ -121,9 +121,10 @@ int swit_plan_set_option(swit_plan *plan, int option, int value);
 
 /* Autotuner (no counterpart in the reference): times `steps` back-to-back launches of
  * each batched step kernel (best of three bursts) on this plan's grid for every candidate forward
- * shots-per-workgroup (1..8) x column-group order and every backward column-group
- * order, and keeps the fastest.  Call between jobs with the model set; saved strips
- * stay valid (the wavefield buffers it uses are cleared by every job).
+ * shots-per-workgroup (1..8) x column-group order (forward with strip saving) and
+ * every backward column-group order, and keeps the fastest.  Call between jobs with
+ * the model set; it uses the wavefield and strip buffers as scratch, so saved strips
+ * (and an exact-adjoint history) are invalidated.
  * choice[3] = {forward shots per workgroup, forward column group, backward column
  * group}; us[2] = the chosen forward / backward step-kernel time in microseconds. */
 int swit_plan_autotune(swit_plan *plan, int steps, int *choice, double *us, hipStream_t stream);

@@ -791,6 +791,8 @@ struct BatchedState {
     float *lam = nullptr;        // [ns][3][nxp][ld] adjoint variables of the exact adjoint
     float *qbuf = nullptr;       // [ns][2][nxp][ld] exact-adjoint temporaries
     float *gk = nullptr;         // [ns][nxp][ld] per-shot gradient with respect to K
+    float *gseis = nullptr;      // [ns][ngmax][nt] copy of dJ/dseis for the captured loop
+    hipGraphExec_t gex_f = nullptr, gex_b = nullptr;   // exact forward / adjoint loops
     bool exact_valid = false;    // dhist holds the last forward_exact of the current model
     int strip_mode = 0;          // 0 FP32, 1 FP16, 2 BF16
     uint16_t *strips16 = nullptr;   // [ns][nt][S] 16-bit strips

@@ -586,6 +586,7 @@ static int batched_init(swit_plan *pl) {
         TRY(pl->alloc(&bs.lam, (size_t)ns * 3 * np));
         TRY(pl->alloc(&bs.qbuf, (size_t)ns * 2 * np));
         TRY(pl->alloc(&bs.gk, (size_t)ns * np));
+        TRY(pl->alloc(&bs.gseis, (size_t)ns * ngmax * d.nt));
     }
     bs.strip_mode = (pl->flags & SWIT_STRIPS_F16) ? 1 : (pl->flags & SWIT_STRIPS_BF16) ? 2 : 0;
     if (bs.strip_mode) {
@@ -619,6 +620,8 @@ static void batched_release(swit_plan *pl) {
     for (auto &g : bs.gfwd)
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
     if (bs.gbwd) { (void)hipGraphExecDestroy(bs.gbwd); bs.gbwd = nullptr; }
+    if (bs.gex_f) { (void)hipGraphExecDestroy(bs.gex_f); bs.gex_f = nullptr; }
+    if (bs.gex_b) { (void)hipGraphExecDestroy(bs.gex_b); bs.gex_b = nullptr; }
     for (auto &t : bs.pending) { (void)hipEventDestroy(t.e0); (void)hipEventDestroy(t.e1); }
     bs.pending.clear();
     for (hipEvent_t e : bs.ev_pool) (void)hipEventDestroy(e);
@@ -854,11 +857,13 @@ static void drop_graphs(BatchedState &bs) {
     for (auto &g : bs.gfwd)
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
     if (bs.gbwd) { (void)hipGraphExecDestroy(bs.gbwd); bs.gbwd = nullptr; }
+    if (bs.gex_f) { (void)hipGraphExecDestroy(bs.gex_f); bs.gex_f = nullptr; }
+    if (bs.gex_b) { (void)hipGraphExecDestroy(bs.gex_b); bs.gex_b = nullptr; }
     ++bs.gen;
 }
 
 // Average duration (us) of `steps` back-to-back step-kernel launches (which = 0 forward
-// without strips, 1 backward) on the plan's buffers, after two warm-up launches.
+// with strip saving, 1 backward) on the plan's buffers, after two warm-up launches.
 static int time_steps(swit_plan *pl, int which, int steps, hipStream_t st, double *us) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
@@ -872,10 +877,10 @@ static int time_steps(swit_plan *pl, int which, int steps, hipStream_t st, doubl
     auto launch = [&](int k) {
         if (which == 0) {
             const int it = 2 + k % (d.nt - 1);
-            switch (bs.strip_mode) {
-            case 0: launch_fwd<0>(pl, it, bs.F[k & 1], bs.F[(k + 1) & 1], 0, st); break;
-            case 1: launch_fwd<1>(pl, it, bs.F[k & 1], bs.F[(k + 1) & 1], 0, st); break;
-            default: launch_fwd<2>(pl, it, bs.F[k & 1], bs.F[(k + 1) & 1], 0, st); break;
+            switch (bs.strip_mode) {   // with strip saving, as the gradient's forward runs
+            case 0: launch_fwd<0>(pl, it, bs.F[k & 1], bs.F[(k + 1) & 1], 1, st); break;
+            case 1: launch_fwd<1>(pl, it, bs.F[k & 1], bs.F[(k + 1) & 1], 1, st); break;
+            default: launch_fwd<2>(pl, it, bs.F[k & 1], bs.F[(k + 1) & 1], 1, st); break;
             }
         } else {
             const int it = d.nt - k % (d.nt - 1);
@@ -913,7 +918,10 @@ extern "C" int swit_plan_autotune(swit_plan *pl, int steps, int *choice, double 
     if (steps < 1 || steps > 1000) return set_err(SWIT_EINVAL, "steps must be in [1, 1000]");
     BatchedState &bs = pl->bat;
     HIP_TRY(hipStreamSynchronize(stream));
-    // the wavefield buffers are scratch here; every job clears what it reads first
+    // the wavefield and strip buffers are scratch here: every job clears what it reads
+    // first, and the saved strips are overwritten
+    pl->strips_valid = false;
+    bs.exact_valid = false;
     static const int kGroups[] = {1, 2, 3, 4, 5, 6, 8};
     static const int kCols[] = {1, 2, 4, 8, 16};
     constexpr int kRepeats = 3;      // each candidate keeps its best of three bursts
@@ -1219,6 +1227,59 @@ extern "C" int swit_misfit(int kind, int ntraces, int nt, float dt, const float 
 
 // ---- exact adjoint ---------------------------------------------------------------------------
 
+static int exact_forward_loop(swit_plan *pl, hipStream_t st) {
+    BatchedState &bs = pl->bat;
+    const Dims &d = pl->d;
+    for (int it = 2; it <= d.nt; ++it) {
+        const float *in = bs.F[it % 2];
+        float *out = bs.F[(it + 1) % 2];
+        switch (bs.strip_mode) {
+        case 0: launch_fwd<0>(pl, it, in, out, 0, st, bs.dhist); break;
+        case 1: launch_fwd<1>(pl, it, in, out, 0, st, bs.dhist); break;
+        default: launch_fwd<2>(pl, it, in, out, 0, st, bs.dhist); break;
+        }
+        LAUNCH_CHECK();
+    }
+    return SWIT_OK;
+}
+
+// Reverse steps nt..2 of the exact adjoint, reading dJ/dseis from bs.gseis.
+static int exact_adjoint_loop(swit_plan *pl, hipStream_t st) {
+    BatchedState &bs = pl->bat;
+    const Dims &d = pl->d;
+    const int ns = pl->ns;
+    const dim3 blk(64, 4), grd(cdiv(d.nzp, 64), cdiv(d.nxp, 4), ns);
+    const dim3 gcol(cdiv(d.nxp, 256), ns), ginj(cdiv(d.ngmax, 256), ns);
+    for (int it = d.nt; it >= 2; --it) {
+        hipLaunchKernelGGL(ex::k_ex_inject, ginj, dim3(256), 0, st, d, it, pl->geo, pl->rec_fwd, bs.gseis, bs.lam);
+        if (d.free_surface == 1) hipLaunchKernelGGL(ex::k_ex_mirror, gcol, dim3(256), 0, st, d, 2, bs.lam);
+        hipLaunchKernelGGL(ex::k_ex_vel, grd, blk, 0, st, d, bs.KB, bs.CV, bs.lam, bs.qbuf);
+        hipLaunchKernelGGL(ex::k_ex_p, grd, blk, 0, st, d, bs.qbuf, bs.lam);
+        if (d.free_surface == 1) hipLaunchKernelGGL(ex::k_ex_mirror, gcol, dim3(256), 0, st, d, 0, bs.lam);
+        hipLaunchKernelGGL(ex::k_ex_grad, grd, blk, 0, st, d, it, bs.KB, bs.CP, bs.dhist, bs.lam, bs.qbuf, bs.gk);
+        hipLaunchKernelGGL(ex::k_ex_vel2, grd, blk, 0, st, d, bs.qbuf, bs.lam);
+        LAUNCH_CHECK();
+    }
+    return SWIT_OK;
+}
+
+static int exact_capture(swit_plan *pl, int which, hipGraphExec_t *exec) {
+    BatchedState &bs = pl->bat;
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(bs.cap, hipStreamCaptureModeThreadLocal));
+    int rc = which == 0 ? exact_forward_loop(pl, bs.cap) : exact_adjoint_loop(pl, bs.cap);
+    hipError_t e = hipStreamEndCapture(bs.cap, &graph);
+    if (rc != SWIT_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return set_err(SWIT_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return set_err(SWIT_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    return SWIT_OK;
+}
+
 extern "C" int swit_forward_exact(swit_plan *pl, const float *src, float *seis, hipStream_t stream) {
     g_err[0] = 0;
     if (!pl || !src || !seis) return set_err(SWIT_EINVAL, "plan, src and seis are required");
@@ -1229,15 +1290,11 @@ extern "C" int swit_forward_exact(swit_plan *pl, const float *src, float *seis, 
     bs.exact_valid = false;
     HIP_TRY(hipMemcpyAsync(bs.src, src, sizeof(float) * pl->ns * d.nt, hipMemcpyDeviceToDevice, stream));
     TRY(forward_prologue(pl, 0, stream));
-    for (int it = 2; it <= d.nt; ++it) {
-        const float *in = bs.F[it % 2];
-        float *out = bs.F[(it + 1) % 2];
-        switch (bs.strip_mode) {
-        case 0: launch_fwd<0>(pl, it, in, out, 0, stream, bs.dhist); break;
-        case 1: launch_fwd<1>(pl, it, in, out, 0, stream, bs.dhist); break;
-        default: launch_fwd<2>(pl, it, in, out, 0, stream, bs.dhist); break;
-        }
-        LAUNCH_CHECK();
+    if (bs.use_graph) {
+        if (!bs.gex_f) TRY(exact_capture(pl, 0, &bs.gex_f));
+        HIP_TRY(hipGraphLaunch(bs.gex_f, stream));
+    } else {
+        TRY(exact_forward_loop(pl, stream));
     }
     hipLaunchKernelGGL(k_transpose, dim3(cdiv(d.ngmax, 32), cdiv(d.nt, 32), pl->ns), dim3(32, 8), 0, stream, d.nt,
                        d.ngmax, pl->seis_t, seis);
@@ -1255,21 +1312,16 @@ extern "C" int swit_adjoint_exact(swit_plan *pl, const float *gseis, double *gra
     const Dims &d = pl->d;
     const int ns = pl->ns;
     const size_t np = (size_t)d.nxp * d.ld;
+    HIP_TRY(hipMemcpyAsync(bs.gseis, gseis, sizeof(float) * ns * d.ngmax * d.nt, hipMemcpyDeviceToDevice, stream));
     TRY(fill(bs.lam, 3 * np * ns, stream));
     TRY(fill(bs.gk, np * ns, stream));
-    const dim3 blk(64, 4), grd(cdiv(d.nzp, 64), cdiv(d.nxp, 4), ns);
-    const dim3 gcol(cdiv(d.nxp, 256), ns), ginj(cdiv(d.ngmax, 256), ns);
-    for (int it = d.nt; it >= 2; --it) {
-        hipLaunchKernelGGL(ex::k_ex_inject, ginj, dim3(256), 0, stream, d, it, pl->geo, pl->rec_fwd, gseis, bs.lam);
-        if (d.free_surface == 1) hipLaunchKernelGGL(ex::k_ex_mirror, gcol, dim3(256), 0, stream, d, 2, bs.lam);
-        hipLaunchKernelGGL(ex::k_ex_vel, grd, blk, 0, stream, d, bs.KB, bs.CV, bs.lam, bs.qbuf);
-        hipLaunchKernelGGL(ex::k_ex_p, grd, blk, 0, stream, d, bs.qbuf, bs.lam);
-        if (d.free_surface == 1) hipLaunchKernelGGL(ex::k_ex_mirror, gcol, dim3(256), 0, stream, d, 0, bs.lam);
-        hipLaunchKernelGGL(ex::k_ex_grad, grd, blk, 0, stream, d, it, bs.KB, bs.CP, bs.dhist, bs.lam, bs.qbuf, bs.gk);
-        hipLaunchKernelGGL(ex::k_ex_vel2, grd, blk, 0, stream, d, bs.qbuf, bs.lam);
-        LAUNCH_CHECK();
+    if (bs.use_graph) {
+        if (!bs.gex_b) TRY(exact_capture(pl, 1, &bs.gex_b));
+        HIP_TRY(hipGraphLaunch(bs.gex_b, stream));
+    } else {
+        TRY(exact_adjoint_loop(pl, stream));
     }
-    hipLaunchKernelGGL(ex::k_ex_fold, dim3(cdiv(d.nz, 64), cdiv(d.nx, 4)), blk, 0, stream, d, ns, pl->c_int,
+    hipLaunchKernelGGL(ex::k_ex_fold, dim3(cdiv(d.nz, 64), cdiv(d.nx, 4)), dim3(64, 4), 0, stream, d, ns, pl->c_int,
                        pl->rho_int, bs.gk, grad);
     LAUNCH_CHECK();
     return SWIT_OK;

@@ -102,7 +102,8 @@ class Plan:
     def autotune(self, steps=20):
         """Time the batched step kernels for every candidate forward shots-per-workgroup
         and column-group order (backward: column-group order) and keep the fastest.
-        Returns {'fwd_group', 'fwd_colgroup', 'bwd_colgroup', 'fwd_us', 'bwd_us'}."""
+        Invalidates saved strips.  Returns {'fwd_group', 'fwd_colgroup', 'bwd_colgroup',
+        'fwd_us', 'bwd_us'}."""
         choice = (ctypes.c_int * 3)()
         us = (ctypes.c_double * 2)()
         _native.check(self._lib.swit_plan_autotune(self._h, int(steps), choice, us, self._stream()))
