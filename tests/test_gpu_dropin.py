@@ -92,3 +92,26 @@ def test_obs_forward_keeps_saved_state_and_syn_without_snapshots_clears_it(tmp_p
     with pytest.raises(ValueError, match='Adjoint solver crash'):
         solver.adjoint(simu, optim)
     solver.release()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme", ["LBFGS", "NLCG"])
+def test_inversion_loop_reduces_misfit(tmp_path, scheme):
+    """Three iterations of the reference inversion loop (swit_amd.inversion) on the
+    drop-in solver: every accepted model lowers the waveform misfit."""
+    from swit_amd import base, inversion, preprocess, solver
+    w = workloads.small(seed=23, nx=60, nz=36, npml=10, nt=300, ns=3)
+    w.rec_xz = w.rec_xz[:, :w.nx]
+    home = str(tmp_path) + "/"
+    simu_true = _simu(home, w, w.vp_true)
+    optim = base.optimize('Waveform', scheme, 3, 0.05, 3500.0, 1400.0, 'Marine', 3, 2, 'None', 1.0, 10.0,
+                          False, 0.0, 'None', False, False, 0.0, 1e9)
+    solver.forward(simu_true, 'obs', savesnap=0)
+    preprocess.process_workflow(simu_true, optim, 'obs')
+    simu = _simu(home, w, w.vp_init)
+    inversion.inversion(simu, optim, {'vp': w.vp_init.astype(np.float64),
+                                      'rho': workloads.gardner(w.vp_init).astype(np.float64)})
+    f = np.loadtxt(home + 'outputs/misfit_data.dat')
+    print(scheme, f)
+    assert len(f) == 4 and np.all(np.diff(f) < 0)
+    solver.release()
