@@ -120,10 +120,13 @@ def main():
                     help="skip the exact-adjoint (torch.autograd) gradient leg")
     ap.add_argument("--cpu-shots", type=int, default=0, help="CPU baseline sample size (0: auto)")
     ap.add_argument("--plain-shots", type=int, default=2, help="plain-path comparison sample (0: skip)")
-    ap.add_argument("--workload", default="c2", choices=["c2", "uneven"],
+    ap.add_argument("--iters", type=int, default=10, help="L-BFGS iterations of --workload lbfgs")
+    ap.add_argument("--workload", default="c2", choices=["c2", "uneven", "lbfgs"],
                     help="c2: the 481x141x4000, 49-shot gradient (default); uneven: three grid "
                          "families (501x171x5000, 481x121x4000, 301x101x3000; 64 shots) as one plan "
-                         "group with the global-correlation misfit on the device")
+                         "group with the global-correlation misfit on the device; lbfgs: --iters "
+                         "L-BFGS iterations of the reference inversion loop through the drop-in solver "
+                         "on the 801x267x6000, 81-shot model with 16-bit strips (--strips, default f16)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (5 times): the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -148,6 +151,8 @@ def main():
 
     if args.workload == "uneven":
         return bench_uneven(args)
+    if args.workload == "lbfgs":
+        return bench_lbfgs(args)
 
     import torch
     from swit_amd import workloads
@@ -266,6 +271,59 @@ def main():
         autograd["over_plain"] = round(autograd["value"] / plain["value"], 2)
     print(json.dumps(line), flush=True)
     plan.close()
+
+
+def bench_lbfgs(args):
+    """L-BFGS through the reference loop (swit_amd.inversion, toolbox/inversion.py) and
+    the drop-in solver, SU files in a RAM-backed folder; 16-bit strips, FP64 shot sum.
+    (The error the 16-bit strips add is measured against the FP64 oracle by
+    tests/test_gpu_strips.py.)"""
+    import shutil
+    import tempfile
+    from swit_amd import base, inversion, preprocess, solver, workloads
+    strips = args.strips if args.strips != "f32" else "f16"
+    w = workloads.config5()
+    root = "/dev/shm" if os.path.isdir("/dev/shm") else None
+    home = tempfile.mkdtemp(prefix="swit_lbfgs_", dir=root) + "/"
+    wav = np.broadcast_to(workloads.ricker(w.nt, w.dt, w.f0), (w.ns, w.nt)).copy()
+
+    def simulate(vp):
+        model = base.model(w.nx, w.nz, w.dx, w.dt, w.nt, True, w.npml, vp.astype(np.float64),
+                           w.rho(vp).astype(np.float64))
+        return base.simulate(model, base.source(w.f0, w.ns, w.src_xz.astype(np.float64), wav),
+                             base.receiver(w.ng, w.rec_xz.astype(np.float64)), base.system(home, 1))
+
+    optim = base.optimize('Waveform', 'LBFGS', args.iters, 0.05, 6000.0, 2000.0, 'Marine', 5, 0, 'None',
+                          1.0, 10.0, False, 0.0, 'None', False, False, 0.0, 1e9)
+    solver.STRIPS = strips
+    t0 = time.perf_counter()
+    simu_true = simulate(w.vp_true)
+    solver.forward(simu_true, 'obs', savesnap=0)
+    preprocess.process_workflow(simu_true, optim, 'obs')
+    t_obs = time.perf_counter() - t0
+    simu = simulate(w.vp_init)
+    t0 = time.perf_counter()
+    inversion.inversion(simu, optim, {'vp': w.vp_init.astype(np.float64),
+                                      'rho': w.rho(w.vp_init).astype(np.float64)})
+    el = time.perf_counter() - t0
+    f = np.loadtxt(home + 'outputs/misfit_data.dat')
+    steps = np.loadtxt(home + 'outputs/line_search_iteration.dat').reshape(-1)
+    solver.release()
+    shutil.rmtree(home, ignore_errors=True)
+
+    line = {
+        "metric": "L-BFGS iterations through the reference inversion loop, 801x267x6000, 81 shots",
+        "value": round(args.iters / el * 60.0, 3), "unit": "iterations/min", "n_gpus": 1,
+        "steps": args.iters, "warmup": 0, "ms_per_step": round(1000 * el / args.iters, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded folded layers 2000-6000 m/s, Ricker 5 Hz)",
+        "config": {"workload": f"{w.nx}x{w.nz} dx {w.dx} npml {w.npml} nt {w.nt} dt {w.dt}, {w.ns} shots, "
+                               f"{w.ng} receivers, waveform misfit, drop-in solver with SU files",
+                   "strips": strips, "cross_shot_sum": "f64"},
+        "misfit": [float(x) for x in f], "line_search_forwards": [int(x) for x in steps],
+        "observed_data_s": round(t_obs, 2),
+    }
+    print(json.dumps(line), flush=True)
 
 
 def bench_uneven(args):
