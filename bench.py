@@ -57,16 +57,16 @@ def load_traffic(kernel):
 def autograd_leg(w, src, obs, vp_init, rho_init, args):
     """The same gradient through torch.autograd with the exact adjoint: one step is
     seismograms(vp) (forward with the divergence history), 1/2 |d - d_obs|^2 and
-    backward (exact adjoint of all shots).  Needs ns * nt * (nx+2npml) * ld * 4 bytes
-    for the history; skipped when the device cannot hold it."""
+    backward (exact adjoint of all shots).  The whole divergence history takes
+    ns * nt * (nx+2npml) * ld * 4 bytes; when the device cannot hold it the plan keeps
+    checkpoints and recomputes segments instead."""
     import torch
     from swit_amd.autograd import l2_misfit, seismograms
     from swit_amd.engine import Plan
     need = w.ns * w.nt * (w.nx + 2 * w.npml) * ((w.nz + 2 * w.npml + 31) // 32 * 32) * 4
     free, _ = torch.cuda.mem_get_info()
-    if need > 0.8 * free:
-        return {"skipped": f"divergence history needs {need / 2**30:.1f} GiB, {free / 2**30:.1f} GiB free"}
-    pe = Plan(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, w.src_xz, w.rec_xz, path="batched", exact=True)
+    mode = True if need <= 0.8 * free else "checkpoint"
+    pe = Plan(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, w.src_xz, w.rec_xz, path="batched", exact=mode)
 
     def step():
         vp = vp_init.clone().requires_grad_(True)
@@ -84,7 +84,8 @@ def autograd_leg(w, src, obs, vp_init, rho_init, args):
     el = time.perf_counter() - t0
     pe.close()
     return {"value": round(w.ns * args.steps / el, 3), "unit": "shot-gradients/s",
-            "ms_per_step": round(1000 * el / args.steps, 2), "history_gib": round(need / 2**30, 1),
+            "ms_per_step": round(1000 * el / args.steps, 2),
+            "history": "full" if mode is True else "checkpoints", "history_gib": round(need / 2**30, 1),
             "finite": bool(torch.isfinite(g).all())}
 
 

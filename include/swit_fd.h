@@ -41,6 +41,11 @@ extern "C" {
 #define SWIT_STRIPS_BF16 0x20   /* boundary strips stored as BF16 with per-step scale */
 #define SWIT_PLAN_EXACT 0x40   /* also allocate the exact adjoint: divergence history of
                                    ns * nt * (nx+2npml) * ld floats */
+#define SWIT_PLAN_EXACT_CKPT 0x80 /* exact adjoint with checkpoints instead: full states
+                                   every c = round(sqrt(nt)) steps and one c-step segment
+                                   of history, about 4 sqrt(nt) * ns * (nx+2npml) * ld
+                                   floats; the reverse sweep recomputes each segment's
+                                   forward (same gradient, bit for bit) */
 
 /* Grid and time axis: the reference's param type (fd2dmpi/datatype.f90:17-22) as
  * read from the parfile (io.f90:72-96).  dx and dt are the float32 values the
@@ -146,7 +151,7 @@ int swit_plan_strips(swit_plan *plan, float *out, hipStream_t stream);
  * uses a different discretisation and is not the exact gradient).  For a misfit J of
  * the seismograms, swit_adjoint_exact turns dJ/dseis into dJ/dvp through
  * K = den c^2 dt/dx and the PML padding; density and the damping profile are held
- * fixed.  Needs a batched plan created with SWIT_PLAN_EXACT.
+ * fixed.  Needs a batched plan created with SWIT_PLAN_EXACT or SWIT_PLAN_EXACT_CKPT.
  *   swit_forward_exact: as swit_forward (same seismograms, bit for bit), also storing
  *     the divergence term of every pressure update (no strips are saved);
  *   swit_adjoint_exact: gseis [ns][ngmax][nt] = dJ/dseis; grad [nx][nz] float64. */

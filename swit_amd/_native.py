@@ -22,6 +22,7 @@ SWIT_PATH_BATCHED = 0x1
 SWIT_STRIPS_F16 = 0x10
 SWIT_STRIPS_BF16 = 0x20
 SWIT_PLAN_EXACT = 0x40
+SWIT_PLAN_EXACT_CKPT = 0x80
 
 # Every entry point include/swit_fd.h declares.
 EXPORTS = (

@@ -10,7 +10,9 @@ The gradient is that of the discrete scheme the forward runs (4th-order staggere
 velocity-pressure, fdcore.f90:75-128), through K = den c^2 dt/dx and the PML padding
 of the model (io.f90:231-300); density and the damping profile (including its vmin)
 are held fixed.  The plan keeps one divergence history, so a backward belongs to the
-last forward through that plan.
+last forward through that plan.  Create the plan with exact=True (whole history,
+ns * nt * (nx+2npml) * ld floats) or exact='checkpoint' (states every ~sqrt(nt) steps,
+segments recomputed in the backward; the same gradient bit for bit).
 """
 from __future__ import annotations
 

@@ -260,6 +260,14 @@ __global__ void k_setup_cvec(Dims d, const float *__restrict__ K, const float *_
 // ---------------------------------------------------------------------------------
 // Forward step it (fdcore.f90:75-160) for a group of G shots per workgroup.
 // In/Out: [ns][3][nxp][ld].  LDS window origin (zo, xo) = (z0-3, x0-3).
+// Divergence history of the exact adjoint: the divergence term of step it of shot s goes
+// to p[(s * slots + it - 1 - base) * np + cell] (all steps: slots = nt, base = 0; one
+// checkpoint segment after state `base`: slots = its length).  p null: not saved.
+struct DSave {
+    float *p;
+    int slots, base;
+};
+
 // strips: null (no saving), FP32 rows (SM = 0) or 16-bit rows (SM = 1, 2) with the
 // FP32 staging rows stage[2][ns][S] and per-row maxima smax[ns][nt] (float bits).
 template <int SM>
@@ -268,8 +276,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
     const ShotGeom *__restrict__ geo, const float *__restrict__ src, const float *__restrict__ In,
     float *__restrict__ Out, const int *__restrict__ rec_start, const RecRef *__restrict__ rec_list,
     float *__restrict__ seis_t, const int *__restrict__ strip_start, const StripRef *__restrict__ strip_list,
-    void *__restrict__ strips, float *__restrict__ stage, uint32_t *__restrict__ smax,
-    float *__restrict__ dsave) {
+    void *__restrict__ strips, float *__restrict__ stage, uint32_t *__restrict__ smax, DSave dsave) {
     using namespace fk;
     __shared__ float sP[kNR];
     __shared__ float sU[kNR];
@@ -365,8 +372,9 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
                                               sW[l + 1], sW[l - 2]);
                     v = cp[c].y * v - cp[c].x * dv;
                     // divergence history for the exact adjoint: tile cells, slot it-1
-                    if (dsave && z >= tb.z0 && z <= tb.z1 && x >= tb.x0 && x <= tb.x1)
-                        dsave[((size_t)s * d.nt + (it - 1)) * np + (unsigned)((x - 1) * d.ld + (z - 1))] = dv;
+                    if (dsave.p && z >= tb.z0 && z <= tb.z1 && x >= tb.x0 && x <= tb.x1)
+                        dsave.p[((size_t)s * dsave.slots + (it - 1 - dsave.base)) * np +
+                                (unsigned)((x - 1) * d.ld + (z - 1))] = dv;
                     if (z == g.isz && x == g.isx) v = v + srcv;
                 }
             }
@@ -787,7 +795,10 @@ struct BatchedState {
     float *end = nullptr;        // saved final forward state [ns][3][nx][nz]
     float *src = nullptr;        // plan-owned copy of the source traces
     float *acc = nullptr;        // [ns][3][nx][nz] per-shot FP32 accumulators
-    float *dhist = nullptr;      // [ns][nt][nxp][ld] divergence history (exact adjoint)
+    float *dhist = nullptr;      // [ns][nt or ck][nxp][ld] divergence history (exact adjoint)
+    int ck = 0;                  // checkpoint interval (0: the whole history is kept)
+    int nck = 0;                 // checkpoints: states after steps 1, 1+ck, 1+2ck, ...
+    float *ckpt = nullptr;       // [nck][ns][3][nxp][ld]
     float *lam = nullptr;        // [ns][3][nxp][ld] adjoint variables of the exact adjoint
     float *qbuf = nullptr;       // [ns][2][nxp][ld] exact-adjoint temporaries
     float *gk = nullptr;         // [ns][nxp][ld] per-shot gradient with respect to K

@@ -48,7 +48,7 @@ __device__ __forceinline__ float at(const Dims &d, const float *f, int iz, int i
     (void)np;                                                         \
     (void)k
 
-// lam: [ns][3][np]; q: [ns][2][np]; gk: [ns][np]; D: [ns][nt][np]
+// lam: [ns][3][np]; q: [ns][2][np]; gk: [ns][np]; D: divergence history (DSave)
 __global__ void k_ex_inject(Dims d, int it, const ShotGeom *__restrict__ geo, const int *__restrict__ rec_cells,
                             const float *__restrict__ g, float *__restrict__ lam) {
     const int s = blockIdx.y;
@@ -103,8 +103,7 @@ __global__ void k_ex_p(Dims d, const float *__restrict__ q, float *__restrict__ 
 }
 
 __global__ void k_ex_grad(Dims d, int it, const float2 *__restrict__ KB, const float2 *__restrict__ CP,
-                          const float *__restrict__ D, float *__restrict__ lam, float *__restrict__ q,
-                          float *__restrict__ gk) {
+                          DSave D, float *__restrict__ lam, float *__restrict__ q, float *__restrict__ gk) {
     EX_CELL;
     float *lp = lam + (size_t)s * 3 * np;
     float *r = q + (size_t)s * 2 * np;
@@ -113,7 +112,7 @@ __global__ void k_ex_grad(Dims d, int it, const float2 *__restrict__ KB, const f
         return;
     }
     const float p = lp[k];
-    gk[(size_t)s * np + k] -= p * D[((size_t)s * d.nt + (it - 1)) * np + k];
+    gk[(size_t)s * np + k] -= p * D.p[((size_t)s * D.slots + (it - 1 - D.base)) * np + k];
     r[k] = KB[k].x * p;
     lp[k] = CP[k].y * p;
 }

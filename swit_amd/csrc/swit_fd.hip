@@ -120,7 +120,8 @@ static int plan_build(swit_plan *pl, const swit_grid *g, const swit_shots *sh, i
     const bool s16 = (flags & (SWIT_STRIPS_F16 | SWIT_STRIPS_BF16)) != 0;
     const bool batched = (flags & SWIT_PATH_BATCHED) != 0;
     if (s16 && !batched) return set_err(SWIT_EINVAL, "16-bit strips need the batched path");
-    if ((flags & SWIT_PLAN_EXACT) && !batched) return set_err(SWIT_EINVAL, "the exact adjoint needs the batched path");
+    if ((flags & (SWIT_PLAN_EXACT | SWIT_PLAN_EXACT_CKPT)) && !batched)
+        return set_err(SWIT_EINVAL, "the exact adjoint needs the batched path");
 
     Dims &d = pl->d;
     d.nx = g->nx;
@@ -581,8 +582,16 @@ static int batched_init(swit_plan *pl) {
     TRY(pl->alloc(&bs.src, (size_t)ns * d.nt));
     TRY(pl->alloc(&bs.acc, (size_t)ns * 3 * ni));
     TRY(pl->alloc(&bs.out, 3 * ni));
-    if (pl->flags & SWIT_PLAN_EXACT) {
-        TRY(pl->alloc(&bs.dhist, (size_t)ns * d.nt * np));
+    if (pl->flags & (SWIT_PLAN_EXACT | SWIT_PLAN_EXACT_CKPT)) {
+        if (pl->flags & SWIT_PLAN_EXACT_CKPT) {
+            // states every ~sqrt(nt) steps, one segment of divergence history
+            bs.ck = std::max(1, (int)std::lround(std::sqrt((double)d.nt)));
+            bs.nck = cdiv(d.nt - 1, bs.ck);
+            TRY(pl->alloc(&bs.ckpt, (size_t)bs.nck * ns * 3 * np));
+            TRY(pl->alloc(&bs.dhist, (size_t)ns * bs.ck * np));
+        } else {
+            TRY(pl->alloc(&bs.dhist, (size_t)ns * d.nt * np));
+        }
         TRY(pl->alloc(&bs.lam, (size_t)ns * 3 * np));
         TRY(pl->alloc(&bs.qbuf, (size_t)ns * 2 * np));
         TRY(pl->alloc(&bs.gk, (size_t)ns * np));
@@ -688,7 +697,7 @@ static int forward_prologue(swit_plan *pl, int save, hipStream_t st) {
 
 template <int SM>
 static void launch_fwd(swit_plan *pl, int it, const float *in, float *out, int save, hipStream_t st,
-                       float *dsave = nullptr) {
+                       DSave dsave = DSave{nullptr, 0, 0}) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
     void *strips = !save ? nullptr : SM == 0 ? static_cast<void *>(pl->strips) : static_cast<void *>(bs.strips16);
@@ -1227,38 +1236,71 @@ extern "C" int swit_misfit(int kind, int ntraces, int nt, float dt, const float 
 
 // ---- exact adjoint ---------------------------------------------------------------------------
 
-static int exact_forward_loop(swit_plan *pl, hipStream_t st) {
+// Forward steps it = a+1..b (state after step a in F[(a+1) % 2]), storing the divergence
+// history as dsave says.
+static int exact_forward_steps(swit_plan *pl, int a, int b, DSave ds, hipStream_t st) {
     BatchedState &bs = pl->bat;
-    const Dims &d = pl->d;
-    for (int it = 2; it <= d.nt; ++it) {
+    for (int it = a + 1; it <= b; ++it) {
         const float *in = bs.F[it % 2];
         float *out = bs.F[(it + 1) % 2];
         switch (bs.strip_mode) {
-        case 0: launch_fwd<0>(pl, it, in, out, 0, st, bs.dhist); break;
-        case 1: launch_fwd<1>(pl, it, in, out, 0, st, bs.dhist); break;
-        default: launch_fwd<2>(pl, it, in, out, 0, st, bs.dhist); break;
+        case 0: launch_fwd<0>(pl, it, in, out, 0, st, ds); break;
+        case 1: launch_fwd<1>(pl, it, in, out, 0, st, ds); break;
+        default: launch_fwd<2>(pl, it, in, out, 0, st, ds); break;
         }
         LAUNCH_CHECK();
     }
     return SWIT_OK;
 }
 
-// Reverse steps nt..2 of the exact adjoint, reading dJ/dseis from bs.gseis.
-static int exact_adjoint_loop(swit_plan *pl, hipStream_t st) {
+static int exact_forward_loop(swit_plan *pl, hipStream_t st) {
+    BatchedState &bs = pl->bat;
+    const Dims &d = pl->d;
+    if (!bs.ck) return exact_forward_steps(pl, 1, d.nt, DSave{bs.dhist, d.nt, 0}, st);
+    // checkpoints: the state after step 1 + j ck (after step 1 it is zero)
+    const size_t chunk = (size_t)pl->ns * 3 * d.nxp * d.ld;
+    TRY(fill(bs.ckpt, chunk, st));
+    for (int j = 1; j < bs.nck; ++j) {
+        const int a = 1 + (j - 1) * bs.ck, b = a + bs.ck;
+        TRY(exact_forward_steps(pl, a, b, DSave{nullptr, 0, 0}, st));
+        TRY(copyk(bs.F[(b + 1) % 2], bs.ckpt + (size_t)j * chunk, chunk, st));
+    }
+    const int a = 1 + (bs.nck - 1) * bs.ck;
+    return exact_forward_steps(pl, a, d.nt, DSave{nullptr, 0, 0}, st);
+}
+
+// Reverse steps b..a+1 of the exact adjoint, reading dJ/dseis from bs.gseis.
+static int exact_adjoint_steps(swit_plan *pl, int a, int b, DSave ds, hipStream_t st) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
     const int ns = pl->ns;
     const dim3 blk(64, 4), grd(cdiv(d.nzp, 64), cdiv(d.nxp, 4), ns);
     const dim3 gcol(cdiv(d.nxp, 256), ns), ginj(cdiv(d.ngmax, 256), ns);
-    for (int it = d.nt; it >= 2; --it) {
+    for (int it = b; it >= a + 1; --it) {
         hipLaunchKernelGGL(ex::k_ex_inject, ginj, dim3(256), 0, st, d, it, pl->geo, pl->rec_fwd, bs.gseis, bs.lam);
         if (d.free_surface == 1) hipLaunchKernelGGL(ex::k_ex_mirror, gcol, dim3(256), 0, st, d, 2, bs.lam);
         hipLaunchKernelGGL(ex::k_ex_vel, grd, blk, 0, st, d, bs.KB, bs.CV, bs.lam, bs.qbuf);
         hipLaunchKernelGGL(ex::k_ex_p, grd, blk, 0, st, d, bs.qbuf, bs.lam);
         if (d.free_surface == 1) hipLaunchKernelGGL(ex::k_ex_mirror, gcol, dim3(256), 0, st, d, 0, bs.lam);
-        hipLaunchKernelGGL(ex::k_ex_grad, grd, blk, 0, st, d, it, bs.KB, bs.CP, bs.dhist, bs.lam, bs.qbuf, bs.gk);
+        hipLaunchKernelGGL(ex::k_ex_grad, grd, blk, 0, st, d, it, bs.KB, bs.CP, ds, bs.lam, bs.qbuf, bs.gk);
         hipLaunchKernelGGL(ex::k_ex_vel2, grd, blk, 0, st, d, bs.qbuf, bs.lam);
         LAUNCH_CHECK();
+    }
+    return SWIT_OK;
+}
+
+// The whole reverse sweep: from the full history, or segment by segment from the
+// checkpoints (each segment's forward is recomputed first, bit for bit).
+static int exact_adjoint_loop(swit_plan *pl, hipStream_t st) {
+    BatchedState &bs = pl->bat;
+    const Dims &d = pl->d;
+    if (!bs.ck) return exact_adjoint_steps(pl, 1, d.nt, DSave{bs.dhist, d.nt, 0}, st);
+    const size_t chunk = (size_t)pl->ns * 3 * d.nxp * d.ld;
+    for (int j = bs.nck - 1; j >= 0; --j) {
+        const int a = 1 + j * bs.ck, b = std::min(a + bs.ck, d.nt);
+        TRY(copyk(bs.ckpt + (size_t)j * chunk, bs.F[(a + 1) % 2], chunk, st));
+        TRY(exact_forward_steps(pl, a, b, DSave{bs.dhist, bs.ck, a}, st));
+        TRY(exact_adjoint_steps(pl, a, b, DSave{bs.dhist, bs.ck, a}, st));
     }
     return SWIT_OK;
 }
@@ -1283,7 +1325,8 @@ static int exact_capture(swit_plan *pl, int which, hipGraphExec_t *exec) {
 extern "C" int swit_forward_exact(swit_plan *pl, const float *src, float *seis, hipStream_t stream) {
     g_err[0] = 0;
     if (!pl || !src || !seis) return set_err(SWIT_EINVAL, "plan, src and seis are required");
-    if (!(pl->flags & SWIT_PLAN_EXACT)) return set_err(SWIT_EINVAL, "plan was not created with SWIT_PLAN_EXACT");
+    if (!(pl->flags & (SWIT_PLAN_EXACT | SWIT_PLAN_EXACT_CKPT)))
+        return set_err(SWIT_EINVAL, "plan was not created with SWIT_PLAN_EXACT or SWIT_PLAN_EXACT_CKPT");
     if (!pl->model_set) return set_err(SWIT_ESTATE, "swit_plan_set_model must be called first");
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
@@ -1306,7 +1349,8 @@ extern "C" int swit_forward_exact(swit_plan *pl, const float *src, float *seis, 
 extern "C" int swit_adjoint_exact(swit_plan *pl, const float *gseis, double *grad, hipStream_t stream) {
     g_err[0] = 0;
     if (!pl || !gseis || !grad) return set_err(SWIT_EINVAL, "plan, gseis and grad are required");
-    if (!(pl->flags & SWIT_PLAN_EXACT)) return set_err(SWIT_EINVAL, "plan was not created with SWIT_PLAN_EXACT");
+    if (!(pl->flags & (SWIT_PLAN_EXACT | SWIT_PLAN_EXACT_CKPT)))
+        return set_err(SWIT_EINVAL, "plan was not created with SWIT_PLAN_EXACT or SWIT_PLAN_EXACT_CKPT");
     BatchedState &bs = pl->bat;
     if (!bs.exact_valid) return set_err(SWIT_ESTATE, "no divergence history: run swit_forward_exact first");
     const Dims &d = pl->d;

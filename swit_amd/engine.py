@@ -60,7 +60,9 @@ class Plan:
         flags = {"plain": _native.SWIT_PATH_PLAIN, "batched": _native.SWIT_PATH_BATCHED}[path]
         flags |= {"f32": 0, "f16": _native.SWIT_STRIPS_F16, "bf16": _native.SWIT_STRIPS_BF16}[strips]
         if exact:
-            flags |= _native.SWIT_PLAN_EXACT
+            # exact=True keeps the whole divergence history; exact='checkpoint' keeps
+            # states every ~sqrt(nt) steps and recomputes segments in the adjoint
+            flags |= _native.SWIT_PLAN_EXACT_CKPT if exact == "checkpoint" else _native.SWIT_PLAN_EXACT
         self.path, self.strips = path, strips
         g = _native.SwitGrid(self.nx, self.nz, self.npml, self.nt, int(free_surface),
                              self.dx, self.dt)
