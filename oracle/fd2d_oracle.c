@@ -10,13 +10,16 @@
  *
  * How each function is pinned (tests/test_oracle_pins.py):
  *   orc_forward_shot   reference dot-product known answer a = 1.25501e35
- *                      (dev/test/02_adjoint_test.ipynb:276), analytic 2-D Green's
- *                      function arrival and amplitude, reciprocity in a heterogeneous
- *                      medium with damping off, impulse response of one step
+ *                      (dev/test/02_adjoint_test.ipynb:276), exact 2-D Green's
+ *                      function solution in a homogeneous medium (waveform and
+ *                      amplitude), reciprocity in a heterogeneous medium with and
+ *                      without damping, one-step impulse response against the
+ *                      textbook 4th-order staggered stencil, source/receiver cell rules
  *   orc_adjoint_shot   known answer ratio = 1.00000 (same notebook), reconstruction
  *                      invariant (reconstructed p == forward p at every step),
- *                      forward illumination == sum of forward p^2, finite-difference
- *                      directional derivative of the L2 misfit (homogeneous medium)
+ *                      forward illumination == sum of forward p^2, kernel against the
+ *                      finite-difference derivative of the L2 misfit (sign, dt scaling)
+ *                      and against the exact discrete gradient (correlation)
  *   orc_setup_pml      closed form of the quadratic profile, corner split, symmetry
  *   orc_pad, orc_vmin  brute force on small arrays
  * Parity unpinned: the density node used by the velocity updates in a heterogeneous
