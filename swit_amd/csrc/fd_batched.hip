@@ -462,8 +462,8 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
 // ---------------------------------------------------------------------------------
 // Backward step it (solver.f90:230-253): reconstruction (fdcore.f90:202-289), adjoint
 // step (fdcore.f90:293-375) and imaging (solver.f90:237-239); one shot per workgroup.
-// The reconstruction window and the adjoint window are staged together and their
-// velocity and pressure updates share barriers.
+// The reconstruction window and the adjoint window are staged together, straight from
+// global memory into LDS, and their velocity and pressure updates share barriers.
 // A thread owns window rows = its lane and the columns 1 + wave + 8j; its tile cells are
 // the owned cells inside the tile, so each cell's {K, B} is loaded once for both the
 // velocity and the pressure pass.  The damping factors (1 - kappa) are loaded only by
@@ -521,27 +521,34 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
     const int cv_end = tb.x1 + 1 - xo;
     const int ct_end = tb.x1 - xo;
 
-    // ---- entry loads: staged windows, coefficients, strip inserts, accumulators ----
-    float rp[kJS], ru[kJS], rw[kJS], bp[kJS], bu[kJS], bw[kJS];
+    // ---- entry loads: both windows straight into LDS (global -> LDS transfers: lane l
+    // of an instruction writes word l after the instruction's base, so no registers hold
+    // the windows), cells outside the grid or the interior zeroed; then coefficients,
+    // strip inserts, accumulators ----
     {
         const float *rin = RIn + (size_t)s * 3 * ni;
         const float *bin = BIn + (size_t)s * 3 * np;
 #pragma unroll
         for (int j = 0; j < kJS; ++j) {
-            int c = wave + kWaves * j;
-            int x = xo + c, jx = x - npml;
-            rp[j] = ru[j] = rw[j] = bp[j] = bu[j] = bw[j] = 0.f;
-            if (c < kCX && rz && jx >= 1 && jx <= nx) {
+            const int c = wave + kWaves * j;
+            if (c >= kCX) continue;
+            const int x = xo + c, jx = x - npml;
+            const int l = LW(c, lane);
+            if (rz && jx >= 1 && jx <= nx) {
                 const unsigned k = (unsigned)((jx - 1) * nz + (jz - 1));
-                rp[j] = rin[k];
-                ru[j] = rin[ni + k];
-                rw[j] = rin[2 * ni + k];
+                __builtin_amdgcn_global_load_lds(rin + k, rP + LW(c, 0), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds(rin + ni + k, rU + LW(c, 0), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds(rin + 2 * ni + k, rW + LW(c, 0), 4, 0, 0);
+            } else {
+                rP[l] = rU[l] = rW[l] = 0.f;
             }
-            if (c < kCX && zin && x >= 1 && x <= d.nxp) {
+            if (zin && x >= 1 && x <= d.nxp) {
                 const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
-                bp[j] = bin[k];
-                bu[j] = bin[np + k];
-                bw[j] = bin[2 * np + k];
+                __builtin_amdgcn_global_load_lds(bin + k, bP + LW(c, 0), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds(bin + np + k, bU + LW(c, 0), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds(bin + 2 * np + k, bW + LW(c, 0), 4, 0, 0);
+            } else {
+                bP[l] = bU[l] = bW[l] = 0.f;
             }
         }
     }
@@ -600,20 +607,8 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
     // the reference's s(it+1) for the reconstruction source subtraction (0 past the end)
     const float ssub = it + 1 <= d.nt ? src[(size_t)s * d.nt + it] : 0.0f;
 
-    // ---- stage both windows ------------------------------------------------------
-#pragma unroll
-    for (int j = 0; j < kJS; ++j) {
-        int c = wave + kWaves * j;
-        if (c < kCX) {
-            int l = LW(c, lane);
-            rP[l] = rp[j];
-            rU[l] = ru[j];
-            rW[l] = rw[j];
-            bP[l] = bp[j];
-            bU[l] = bu[j];
-            bW[l] = bw[j];
-        }
-    }
+    // ---- both windows staged: every wave's transfers complete, then the barrier ----
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     // ---- velocities: reconstruction u', w' (:221-247), adjoint u', w' (:310-326) ----
 #pragma unroll
