@@ -122,20 +122,25 @@ const char *swit_last_error(void);
 #define SWIT_OPT_FWD_GROUP 4    /* shots per workgroup of the forward step kernel     */
 #define SWIT_OPT_FWD_COLGROUP 8 /* tile columns per group of the forward work order   */
 #define SWIT_OPT_BWD_COLGROUP 16 /* tile columns per group of the backward work order */
+#define SWIT_OPT_BWD_STEPS 32   /* 2 (default): backward kernel doing two time steps per
+                                   launch; 1: one step per launch (same results)      */
 int swit_plan_set_option(swit_plan *plan, int option, int value);
 
 /* Autotuner (no counterpart in the reference): times `steps` back-to-back launches of
  * each batched step kernel (best of three bursts) on this plan's grid for every candidate forward
  * shots-per-workgroup (1..8) x column-group order (forward with strip saving) and
- * every backward column-group order, and keeps the fastest.  Call between jobs with
- * the model set; it uses the wavefield and strip buffers as scratch, so saved strips
- * (and an exact-adjoint history) are invalidated.
- * choice[3] = {forward shots per workgroup, forward column group, backward column
- * group}; us[2] = the chosen forward / backward step-kernel time in microseconds. */
+ * every backward kernel (one or two steps per launch) x column-group order, and keeps
+ * the fastest per time step.  Call between jobs with the model set; it uses the
+ * wavefield and strip buffers as scratch, so saved strips (and an exact-adjoint
+ * history) are invalidated.
+ * choice[4] = {forward shots per workgroup, forward column group, backward column
+ * group, backward steps per launch}; us[2] = the chosen forward / backward step-kernel
+ * time per time step in microseconds. */
 int swit_plan_autotune(swit_plan *plan, int steps, int *choice, double *us, hipStream_t stream);
 
-/* Accumulated duration and launch count of the step kernels timed while
- * SWIT_OPT_PROFILE was on: which = 0 forward step kernel, 1 backward step kernel.
+/* Accumulated duration and time-step count of the step kernels timed while
+ * SWIT_OPT_PROFILE was on: which = 0 forward step kernel, 1 backward step kernel
+ * (a two-step backward launch counts two steps).
  * Each time loop is bracketed by one pair of events (the step kernels run back to
  * back in between); this call waits for the loops recorded so far to finish. */
 int swit_plan_kernel_time(swit_plan *plan, int which, double *total_ms, int64_t *launches);

@@ -53,6 +53,7 @@ SWIT_OPT_PROFILE = 2
 SWIT_OPT_FWD_GROUP = 4
 SWIT_OPT_FWD_COLGROUP = 8
 SWIT_OPT_BWD_COLGROUP = 16
+SWIT_OPT_BWD_STEPS = 32
 
 
 class SwitGrid(ctypes.Structure):

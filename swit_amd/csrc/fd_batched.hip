@@ -80,7 +80,7 @@ struct TileBox {
     int z0, z1, x0, x1;
 };
 
-__device__ __forceinline__ TileBox tile_box(const Dims &d, const TileMap &tm, int t) {
+__host__ __device__ inline TileBox tile_box(const Dims &d, const TileMap &tm, int t) {
     int zt = t % tm.nzt, xt = t / tm.nzt;
     TileBox b;
     b.z0 = max(tm.ztop + zt * tm.th, 1);
@@ -734,6 +734,313 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
     }
 }
 
+// ---------------------------------------------------------------------------------
+// Two backward steps per launch: steps it and it-1 (solver.f90:230-253), each doing
+// exactly the per-cell arithmetic of k_bwd_fused, so one launch here and two launches of
+// k_bwd_fused give the same bits.  The window has a margin of 6 cells, 3 for each step:
+// step 1 updates the window minus 3 cells per side (velocities minus their stencils'
+// reach), step 2 the 52x16 tile, from step 1's values in LDS.  The three fields of both
+// wavefields and the accumulators therefore cross HBM once per two steps.
+// Window rows are lanes and a thread's columns are wave - 2 + 8j (j < 4), so the tile
+// columns are exactly its slots j = 1, 2 and each owned cell's coefficients serve both
+// steps.
+// inj_*: injection groups of the tile (step 2), per-shot capacity ngmax; injx_*: those
+// of the step-1 region (the tile plus 3 cells per side), per-shot capacity injx_cap.
+namespace bk2 {
+constexpr int kZL = 64;
+constexpr int kM = 6;                  // margin: 3 cells per step
+constexpr int kTZ = kZL - 2 * kM;      // 52 output rows
+constexpr int kCX = kTX + 2 * kM;      // 28 window columns
+constexpr int kNL = kZL * kCX;         // floats per LDS array
+constexpr int kJ = 4;                  // column slots of a thread
+__device__ __forceinline__ int LW(int c, int l) { return c * kZL + l; }
+}  // namespace bk2
+
+template <int SM>
+__global__ __launch_bounds__(kThreads) void k_bwd_fused2(
+    Dims d, int it, TileMap tm, const float2 *__restrict__ KB, const float2 *__restrict__ CP,
+    const float4 *__restrict__ CV,
+    const ShotGeom *__restrict__ geo, const float *__restrict__ src, const void *__restrict__ strips,
+    const uint32_t *__restrict__ smax,
+    const float *__restrict__ RIn, float *__restrict__ ROut, const float *__restrict__ BIn,
+    float *__restrict__ BOut, const int *__restrict__ inj_gstart, const InjGroup *__restrict__ inj_groups,
+    const RecRef *__restrict__ inj_list, const int *__restrict__ injx_gstart,
+    const InjGroup *__restrict__ injx_groups, const RecRef *__restrict__ injx_list, int injx_cap,
+    const float *__restrict__ sadj_t, float *__restrict__ acc) {
+    using namespace bk2;
+    __shared__ float rP[kNL];
+    __shared__ float rU[kNL];
+    __shared__ float rW[kNL];
+    __shared__ float bP[kNL];
+    __shared__ float bU[kNL];
+    __shared__ float bW[kNL];
+
+    int s, t;
+    work_item(xcd_logical(blockIdx.x, gridDim.x), (int)(gridDim.x / tm.ntiles), tm, s, t);
+    const TileBox tb = tile_box(d, tm, t);
+    const int zo = tb.z0 - kM, xo = tb.x0 - kM;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int fs = d.fs, npml = d.npml, nx = d.nx, nz = d.nz;
+    const int np = d.nxp * d.ld, ni = nx * nz;
+    const int ez = tb.z1 - zo, ex = tb.x1 - xo;      // last tile row / column of the window
+    const bool has_interior = max(zo, npml + 1) <= min(zo + kZL - 1, npml + nz) &&
+                              max(xo, npml + 1) <= min(xo + kCX - 1, npml + nx);
+    const int top0 = 3 * nx + 6 * nz;
+    // updated cells span window rows 1..ez+4 and columns 1..ex+4; the velocity factors
+    // also read damping at the next row / column
+    const bool damped = !(zo + 1 >= npml + 1 && zo + ez + 5 <= npml + nz && xo + 1 >= npml + 1 &&
+                          xo + ex + 5 <= npml + nx);
+    const float *Pc = reinterpret_cast<const float *>(CP);
+    const ShotGeom g = geo[s];
+    const StripRow<SM> b1(strips, (size_t)s * d.nt + (it - 2), d.S, smax);
+    const StripRow<SM> b2(strips, (size_t)s * d.nt + (it - 3), d.S, smax);
+    const int z = zo + lane, jz = z - npml;
+    const bool zin = z >= 1 && z <= d.nzp;
+    const bool rz = has_interior && jz >= 1 && jz <= nz;
+    const bool zu = z >= 2 && z <= d.nzp - 1;                  // adjoint u rows (:311)
+    const bool zw = z >= 2 && z <= d.nzp - 2;                  // adjoint w rows (:320)
+    const bool zp = z >= 3 && z <= d.nzp - 2;                  // adjoint p rows (:337)
+    // rows of the two steps: velocities v*, pressures p* (p2 = the tile rows)
+    const bool v1 = lane >= 1 && lane <= ez + 4 && zin, p1 = lane >= 3 && lane <= ez + 3 && zin;
+    const bool v2 = lane >= 4 && lane <= ez + 1 && zin, p2 = lane >= kM && lane <= ez;
+    constexpr int kJS = (kCX + kWaves - 1) / kWaves;
+
+    // ---- entry loads: both windows straight into LDS, then coefficients, strip inserts
+    // of both steps, accumulators ----
+    {
+        const float *rin = RIn + (size_t)s * 3 * ni;
+        const float *bin = BIn + (size_t)s * 3 * np;
+#pragma unroll
+        for (int j = 0; j < kJS; ++j) {
+            const int c = wave + kWaves * j;
+            if (c >= kCX) continue;
+            const int x = xo + c, jx = x - npml;
+            const int l = LW(c, lane);
+            if (rz && jx >= 1 && jx <= nx) {
+                const unsigned k = (unsigned)((jx - 1) * nz + (jz - 1));
+                __builtin_amdgcn_global_load_lds(rin + k, rP + LW(c, 0), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds(rin + ni + k, rU + LW(c, 0), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds(rin + 2 * ni + k, rW + LW(c, 0), 4, 0, 0);
+            } else {
+                rP[l] = rU[l] = rW[l] = 0.f;
+            }
+            if (zin && x >= 1 && x <= d.nxp) {
+                const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
+                __builtin_amdgcn_global_load_lds(bin + k, bP + LW(c, 0), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds(bin + np + k, bU + LW(c, 0), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds(bin + 2 * np + k, bW + LW(c, 0), 4, 0, 0);
+            } else {
+                bP[l] = bU[l] = bW[l] = 0.f;
+            }
+        }
+    }
+    // owned cells: kb = {K, B}, fac = {1-kappa_p, 1-kappa_u, 1-kappa_w}; strip values
+    // inserted into the reconstructed u, w (su*, sw*) and p (sp*) at steps 1 and 2;
+    // on tile cells ta = this shot's accumulators
+    float2 kb[kJ];
+    float fp[kJ], fu[kJ], fw[kJ], su1[kJ], sw1[kJ], su2[kJ], sw2[kJ], sp1[kJ];
+    float sp2[2], ta0[2], ta1[2], ta2[2];
+    float *a = acc + (size_t)s * 3 * ni;
+#pragma unroll
+    for (int j = 0; j < kJ; ++j) {
+        const int c = wave - 2 + kWaves * j;
+        const int x = xo + c, jx = x - npml;
+        kb[j] = make_float2(0.f, 0.f);
+        fp[j] = fu[j] = fw[j] = 1.0f;
+        su1[j] = sw1[j] = su2[j] = sw2[j] = sp1[j] = 0.f;
+        if (j == 1 || j == 2) sp2[j - 1] = ta0[j - 1] = ta1[j - 1] = ta2[j - 1] = 0.f;
+        if (c < 1 || c > ex + 4 || !v1 || x < 1 || x > d.nxp) continue;
+        const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
+        kb[j] = KB[k];
+        if (damped) {
+            const float4 cv = CV[k];
+            fu[j] = cv.y;
+            fw[j] = cv.z;
+            fp[j] = Pc[2 * k + 1];
+        }
+        if (!(rz && jx >= 1 && jx <= nx)) continue;
+        if (jx == 1 || jx == nx - 1) {
+            const int q = 3 * nx + 6 * (jz - 1) + (jx == 1 ? 2 : 5);
+            su1[j] = b1[q];
+            su2[j] = b2[q];
+        }
+        if (jz == nz - 1) {
+            const int q = 3 * (jx - 1) + 2;
+            sw1[j] = b1[q];
+            sw2[j] = b2[q];
+        }
+        int q = -1;
+        if (jz <= 3) q = top0 + 6 * (jx - 1) + (jz - 1);
+        else if (jx <= 2 || jx >= nx - 1) q = 3 * nx + 6 * (jz - 1) + (jx == 1 ? 0 : jx == 2 ? 1 : jx == nx ? 3 : 4);
+        else if (jz >= nz - 1) q = 3 * (jx - 1) + (jz == nz ? 0 : 1);
+        if (q >= 0 && c >= 3 && c <= ex + 3 && p1) sp1[j] = b1[q];
+        if ((j == 1 || j == 2) && c <= ex && p2) {
+            if (q >= 0) sp2[j - 1] = b2[q];
+            const unsigned ki = (unsigned)((jx - 1) * nz + (jz - 1));
+            ta0[j - 1] = a[ki];
+            ta1[j - 1] = a[ni + ki];
+            ta2[j - 1] = a[2 * ni + ki];
+        }
+    }
+    // the reference's s(it+1) of each step's source subtraction (0 past the end)
+    const float ssub1 = it + 1 <= d.nt ? src[(size_t)s * d.nt + it] : 0.0f;
+    const float ssub2 = src[(size_t)s * d.nt + it - 1];
+    const bool src_in = g.isx_i > 2 && g.isx_i < nx - 1 && g.isz_i > 3 && g.isz_i < nz - 1;
+    const int m = fs - 1 - zo;                         // window row of the mirror row fs-1
+
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+
+    // One step's velocities (reconstruction :221-247, adjoint :310-326) on columns
+    // [clo, chi] of the rows vr, then the adjoint w mirror (:329-333) if rows m, m+1 are
+    // velocity rows (m+1 <= vhi).
+    auto velocities = [&](bool vr, int clo, int chi, int vhi, const float *su, const float *sw) {
+#pragma unroll
+        for (int j = 0; j < kJ; ++j) {
+            const int c = wave - 2 + kWaves * j;
+            if (c < clo || c > chi || !vr) continue;
+            const int x = xo + c, jx = x - npml;
+            if (x < 1 || x > d.nxp) continue;
+            const int l = LW(c, lane);
+            if (rz && jx >= 1 && jx <= nx) {
+                if (jx == 1 || jx == nx - 1) rU[l] = su[j];
+                else if (jx <= nx - 2 && jz >= 2 && jz <= nz - 1)
+                    rU[l] = v_undo(rU[l], kb[j].y, rP[l], rP[l + kZL], rP[l + 2 * kZL], rP[l - kZL]);
+                if (jz == nz - 1) rW[l] = sw[j];
+                else if (jx >= 2 && jx <= nx - 1 && jz >= 2 && jz <= nz - 2)
+                    rW[l] = v_undo(rW[l], kb[j].y, rP[l], rP[l + 1], rP[l + 2], rP[l - 1]);
+            }
+            if (zu && x >= 2 && x <= d.nxp - 2)
+                bU[l] = v_update_f(bU[l], kb[j].x, fu[j], bP[l], bP[l + kZL], bP[l + 2 * kZL], bP[l - kZL]);
+            if (zw && x >= 2 && x <= d.nxp - 1)
+                bW[l] = v_update_f(bW[l], kb[j].x, fw[j], bP[l], bP[l + 1], bP[l + 2], bP[l - 1]);
+        }
+        __syncthreads();
+        if (d.free_surface == 1 && lane == m && m >= 1 && m + 1 <= vhi) {
+#pragma unroll
+            for (int j = 0; j < kJ; ++j) {
+                const int c = wave - 2 + kWaves * j;
+                const int x = xo + c;
+                if (c >= clo && c <= chi && x >= 2 && x <= d.nxp - 1) bW[LW(c, m)] = bW[LW(c, m + 1)];
+            }
+        }
+        __syncthreads();
+    };
+    // One step's pressures (reconstruction :250-287, adjoint :336-345) on columns
+    // [clo, chi] of the rows pr; both are updated in place (a thread reads rP and bP only
+    // at its own cells in this pass).  Returns the reconstructed and the old adjoint p of
+    // the tile slots in prn / pold.
+    auto pressures = [&](bool pr, int clo, int chi, const float *sp, const float *sp_t, float ssub, float *prn,
+                         float *pold) {
+#pragma unroll
+        for (int j = 0; j < kJ; ++j) {
+            const int c = wave - 2 + kWaves * j;
+            if (c < clo || c > chi || !pr) continue;
+            const int x = xo + c, jx = x - npml;
+            if (x < 1 || x > d.nxp) continue;
+            const int l = LW(c, lane);
+            if (rz && jx >= 1 && jx <= nx) {
+                float v;
+                if (jz <= 3 || jx <= 2 || jx >= nx - 1 || jz >= nz - 1) {
+                    v = sp_t ? ((j == 1 || j == 2) ? sp_t[j - 1] : 0.f) : sp[j];
+                } else {
+                    v = rP[l];
+                    if (jz == g.isz_i && jx == g.isx_i && src_in) v = v - d.dt * ssub;
+                    v = p_undo(v, kb[j].x, rU[l], rU[l - kZL], rU[l + kZL], rU[l - 2 * kZL],
+                               rW[l], rW[l - 1], rW[l + 1], rW[l - 2]);
+                }
+                rP[l] = v;
+                if (j == 1 || j == 2) prn[j - 1] = v;
+            }
+            float v = bP[l];
+            if (j == 1 || j == 2) pold[j - 1] = v;
+            if (zp && x >= 3 && x <= d.nxp - 2)
+                v = p_update_f(v, kb[j].y, fp[j], bU[l], bU[l - kZL], bU[l + kZL], bU[l - 2 * kZL],
+                               bW[l], bW[l - 1], bW[l + 1], bW[l - 2]);
+            bP[l] = v;
+        }
+        __syncthreads();
+    };
+    // Adjoint-source injection (:348-367) of one step's region, receivers sharing a cell
+    // added in receiver order, then the p mirror (:370-373) on columns [clo, chi] when
+    // row m+2 is a pressure row of the step (plo <= m+2 <= phi).
+    auto inject_mirror = [&](const int *gst, const InjGroup *grp, const RecRef *lst, int cap, int row, int clo,
+                             int chi, int plo, int phi) {
+        const int g0 = gst[(size_t)s * (tm.ntiles + 1) + t];
+        const int g1 = gst[(size_t)s * (tm.ntiles + 1) + t + 1];
+        const RecRef *el = lst + (size_t)s * cap;
+        const float *sa = sadj_t + ((size_t)s * d.nt + row) * d.ngmax;
+        for (int q = g0 + (int)threadIdx.x; q < g1; q += kThreads) {
+            const InjGroup gr = grp[(size_t)s * cap + q];
+            const RecRef r0 = el[gr.e0];
+            const int l = LW(r0.ix - xo, r0.iz - zo);
+            float v = bP[l];
+            for (int e = gr.e0; e < gr.e1; ++e) v = v + d.dt * sa[el[e].idx];
+            bP[l] = v;
+        }
+        __syncthreads();
+        if (lane == m && m >= 0 && m + 2 >= plo && m + 2 <= phi) {
+#pragma unroll
+            for (int j = 0; j < kJ; ++j) {
+                const int c = wave - 2 + kWaves * j;
+                const int x = xo + c;
+                if (c >= clo && c <= chi && x >= 1 && x <= d.nxp) bP[LW(c, m)] = -bP[LW(c, m + 2)];
+            }
+        }
+        __syncthreads();
+    };
+
+    // ---- step 1 (it): window minus 3 cells per side ----
+    float pr1[2] = {0.f, 0.f}, po1[2] = {0.f, 0.f};
+    velocities(v1, 1, ex + 4, ez + 4, su1, sw1);
+    pressures(p1, 3, ex + 3, sp1, nullptr, ssub1, pr1, po1);
+    inject_mirror(injx_gstart, injx_groups, injx_list, injx_cap, it - 1, 3, ex + 3, 3, ez + 3);
+    // step 1 imaging on the tile: dg += p_rec (p0 - p), dfw += p_rec^2, dbk += p^2
+#pragma unroll
+    for (int j = 1; j <= 2; ++j) {
+        const int c = wave - 2 + kWaves * j;
+        const int jx = xo + c - npml;
+        if (c > ex || !p2 || !(rz && jx >= 1 && jx <= nx)) continue;
+        const float pb = bP[LW(c, lane)];
+        const float pr = pr1[j - 1];
+        ta0[j - 1] = ta0[j - 1] + pr * (po1[j - 1] - pb);
+        ta1[j - 1] = ta1[j - 1] + pr * pr;
+        ta2[j - 1] = ta2[j - 1] + pb * pb;
+    }
+    // ---- step 2 (it-1): the tile ----
+    float pr2[2] = {0.f, 0.f}, po2[2] = {0.f, 0.f};
+    velocities(v2, 4, ex + 1, ez + 1, su2, sw2);
+    pressures(p2, kM, ex, nullptr, sp2, ssub2, pr2, po2);
+    inject_mirror(inj_gstart, inj_groups, inj_list, d.ngmax, it - 2, kM, ex, kM, ez);
+
+    // ---- write back; step 2 imaging ----
+    float *bout = BOut + (size_t)s * 3 * np;
+    float *rout = ROut + (size_t)s * 3 * ni;
+#pragma unroll
+    for (int j = 1; j <= 2; ++j) {
+        const int c = wave - 2 + kWaves * j;
+        if (c > ex || !p2) continue;
+        const int x = xo + c, jx = x - npml;
+        const int l = LW(c, lane);
+        const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
+        const float pb = bP[l];
+        bout[k] = pb;
+        bout[np + k] = bU[l];
+        bout[2 * np + k] = bW[l];
+        if (rz && jx >= 1 && jx <= nx) {
+            const unsigned ki = (unsigned)((jx - 1) * nz + (jz - 1));
+            const float pr = pr2[j - 1];
+            rout[ki] = pr;
+            rout[ni + ki] = rU[l];
+            rout[2 * ni + ki] = rW[l];
+            a[ki] = ta0[j - 1] + pr * (po2[j - 1] - pb);
+            a[ni + ki] = ta1[j - 1] + pr * pr;
+            a[2 * ni + ki] = ta2[j - 1] + pb * pb;
+        }
+    }
+}
+
 // Cross-shot FP64 sum of the per-shot kernels and illuminations
 // (solver.f90:263-265) and g = -2 g / c (solver.f90:270).  acc: [ns][3][nx][nz].
 __global__ void k_reduce_acc(Dims d, int ns, const float *__restrict__ vp, const float *__restrict__ acc,
@@ -779,6 +1086,17 @@ struct BatchedState {
     int *inj_start = nullptr;    // [ns][bwd.ntiles+1] group ranges per tile
     InjGroup *inj_groups = nullptr;  // [ns][ngmax]
     RecRef *inj_list = nullptr;  // [ns][ngmax]
+    // two-step backward kernel: its tiling, the injection groups of its tiles (step 2)
+    // and of its tiles grown by 3 cells per side (step 1, per-shot capacity injx_cap)
+    TileMap bwd2{};
+    int bwd_steps = 2;           // backward steps per launch: 1 or 2
+    int *inj2_start = nullptr;
+    InjGroup *inj2_groups = nullptr;
+    RecRef *inj2_list = nullptr;
+    int *injx_start = nullptr;
+    InjGroup *injx_groups = nullptr;
+    RecRef *injx_list = nullptr;
+    int injx_cap = 0;
     int *strip_start = nullptr;  // [fwd.ntiles+1]
     StripRef *strip_list = nullptr;
     float2 *KB = nullptr;        // [nxp][ld] {K, B}

@@ -470,6 +470,80 @@ static TileMap backward_tilemap(const Dims &d) {
     return best;
 }
 
+// Two-step backward tiling: 52-row tiles whose windows reach 6 rows past each side.
+// Within a window the mirror row fs-1 must not sit where a step needs it while its
+// source row (fs for w, fs+1 for p) lies past that step's updated rows, which is the
+// case for fs-1 in [z1-1, z1+4] of a tile ending at row z1.  Rows 2..nzp-1 are
+// covered; pick the offset with the fewest tiles.
+static TileMap backward2_tilemap(const Dims &d) {
+    const int th = bk2::kTZ, fs = d.fs;
+    TileMap best{};
+    best.ntiles = 1 << 30;
+    for (int shift = 0; shift < th; ++shift) {
+        TileMap tm;
+        tm.ztop = 2 - shift;
+        tm.th = th;
+        tm.nzt = cdiv(d.nzp - 1 - tm.ztop + 1, th);
+        tm.nxt = cdiv(d.nxp, kTX);
+        tm.ntiles = tm.nzt * tm.nxt;
+        tm.cg = std::min(kColGroup, tm.nxt);
+        bool ok = true;
+        for (int zt = 0; zt < tm.nzt; ++zt) {
+            const int z1 = std::min(tm.ztop + (zt + 1) * th - 1, d.nzp);
+            if (fs - 1 >= z1 - 1 && fs - 1 <= z1 + 4) ok = false;
+        }
+        if (ok && tm.ntiles < best.ntiles) best = tm;
+    }
+    return best;
+}
+
+// Injection groups per tile: for every shot, the receivers whose injection cell lies in
+// the tile grown by `margin` cells per side, listed tile by tile, sorted by cell within
+// a tile (stably, so receivers sharing a cell stay in receiver order) and grouped by
+// cell, so one thread adds every receiver of a cell in order.
+// gst: [ns][ntiles+1] group ranges per tile; groups, list: [ns][cap].
+static int build_injection(swit_plan *pl, const TileMap &tm, int margin, int cap, std::vector<int> &gst,
+                           std::vector<InjGroup> &groups, std::vector<RecRef> &list) {
+    const Dims &d = pl->d;
+    const int ns = pl->ns, ngmax = d.ngmax;
+    gst.assign((size_t)ns * (tm.ntiles + 1), 0);
+    groups.assign((size_t)ns * cap, InjGroup{0, 0});
+    list.assign((size_t)ns * cap, RecRef{0, 0, 0, 0});
+    for (int s = 0; s < ns; ++s) {
+        std::vector<RecRef> rec;
+        for (int r = 0; r < pl->geo_h[s].ng; ++r) {
+            const int ci = pl->rec_cells_inj[(size_t)s * ngmax + r];
+            rec.push_back(RecRef{r, ci % d.ld + 1, ci / d.ld + 1, 0});
+        }
+        RecRef *el = &list[(size_t)s * cap];
+        InjGroup *gr = &groups[(size_t)s * cap];
+        int *gs = &gst[(size_t)s * (tm.ntiles + 1)];
+        int ne = 0, ng = 0;
+        for (int t = 0; t < tm.ntiles; ++t) {
+            const TileBox b = tile_box(d, tm, t);
+            const int e0 = ne;
+            for (const RecRef &r : rec) {
+                if (r.iz < b.z0 - margin || r.iz > b.z1 + margin || r.ix < b.x0 - margin || r.ix > b.x1 + margin)
+                    continue;
+                if (ne == cap) return set_err(SWIT_EINVAL, "shot %d: more than %d injection entries", s, cap);
+                el[ne++] = r;
+            }
+            std::stable_sort(el + e0, el + ne, [](const RecRef &x, const RecRef &y) {
+                return x.ix != y.ix ? x.ix < y.ix : x.iz < y.iz;
+            });
+            gs[t] = ng;
+            for (int e = e0; e < ne;) {
+                int f = e + 1;
+                while (f < ne && el[f].ix == el[e].ix && el[f].iz == el[e].iz) ++f;
+                gr[ng++] = InjGroup{e, f};
+                e = f;
+            }
+        }
+        gs[tm.ntiles] = ng;
+    }
+    return SWIT_OK;
+}
+
 static int fill(float *p, size_t n, hipStream_t st) {
     int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
     if (blocks < 1) blocks = 1;
@@ -492,57 +566,36 @@ static int batched_init(swit_plan *pl) {
     const int ns = pl->ns, ngmax = d.ngmax, fs = d.fs;
     bs.fwd = make_tilemap(d, fs - 1, fk::kTZ);
     bs.bwd = backward_tilemap(d);
-    // receivers: recording cells (forward tiling) and injection cells (backward tiling)
-    std::vector<int> rs((size_t)ns * (bs.fwd.ntiles + 1)), is((size_t)ns * (bs.bwd.ntiles + 1));
-    std::vector<RecRef> rl((size_t)ns * ngmax), il((size_t)ns * ngmax);
-    std::vector<InjGroup> ig((size_t)ns * ngmax, InjGroup{0, 0});
+    bs.bwd2 = backward2_tilemap(d);
+    // receivers: recording cells by forward tile
+    std::vector<int> rs((size_t)ns * (bs.fwd.ntiles + 1));
+    std::vector<RecRef> rl((size_t)ns * ngmax);
     for (int s = 0; s < ns; ++s) {
-        const ShotGeom &g = pl->geo_h[s];
-        (void)g;
-        std::vector<std::pair<int, RecRef>> a, b;
+        std::vector<std::pair<int, RecRef>> a;
         for (int r = 0; r < pl->geo_h[s].ng; ++r) {
-            int cf = pl->rec_cells_fwd[(size_t)s * ngmax + r], ci = pl->rec_cells_inj[(size_t)s * ngmax + r];
-            RecRef rf{r, cf % d.ld + 1, cf / d.ld + 1, 0}, ri{r, ci % d.ld + 1, ci / d.ld + 1, 0};
+            int cf = pl->rec_cells_fwd[(size_t)s * ngmax + r];
+            RecRef rf{r, cf % d.ld + 1, cf / d.ld + 1, 0};
             if (rf.iz < fs - 1) return set_err(SWIT_EINVAL, "shot %d receiver %d above the free surface", s, r);
             a.push_back({tile_of(bs.fwd, rf.iz, rf.ix), rf});
-            b.push_back({tile_of(bs.bwd, ri.iz, ri.ix), ri});
         }
         auto by_tile = [](const std::pair<int, RecRef> &x, const std::pair<int, RecRef> &y) { return x.first < y.first; };
         std::stable_sort(a.begin(), a.end(), by_tile);
-        std::stable_sort(b.begin(), b.end(), by_tile);
-        int *sa = &rs[(size_t)s * (bs.fwd.ntiles + 1)], *sb = &is[(size_t)s * (bs.bwd.ntiles + 1)];
+        int *sa = &rs[(size_t)s * (bs.fwd.ntiles + 1)];
         for (int t = 0, k = 0; t <= bs.fwd.ntiles; ++t) {
             while (k < (int)a.size() && a[k].first < t) ++k;
             sa[t] = k;
         }
-        for (int t = 0, k = 0; t <= bs.bwd.ntiles; ++t) {
-            while (k < (int)b.size() && b[k].first < t) ++k;
-            sb[t] = k;
-        }
         for (size_t k = 0; k < a.size(); ++k) rl[(size_t)s * ngmax + k] = a[k].second;
-        for (size_t k = 0; k < b.size(); ++k) il[(size_t)s * ngmax + k] = b[k].second;
-        // injection entries of a tile sorted by cell (stable: receiver order kept),
-        // then grouped so one thread adds every receiver of a cell in order
-        std::vector<InjGroup> grs;
-        std::vector<int> ent(sb, sb + bs.bwd.ntiles + 1);   // entry ranges per tile
-        int *gs = sb;                                        // becomes group ranges per tile
-        for (int t = 0; t < bs.bwd.ntiles; ++t) {
-            int e0 = ent[t], e1 = ent[t + 1];
-            RecRef *base = &il[(size_t)s * ngmax];
-            std::stable_sort(base + e0, base + e1, [](const RecRef &x, const RecRef &y) {
-                return x.ix != y.ix ? x.ix < y.ix : x.iz < y.iz;
-            });
-            gs[t] = (int)grs.size();
-            for (int e = e0; e < e1;) {
-                int f = e + 1;
-                while (f < e1 && base[f].ix == base[e].ix && base[f].iz == base[e].iz) ++f;
-                grs.push_back(InjGroup{e, f});
-                e = f;
-            }
-        }
-        gs[bs.bwd.ntiles] = (int)grs.size();
-        for (size_t k = 0; k < grs.size(); ++k) ig[(size_t)s * ngmax + k] = grs[k];
     }
+    // injection cells: tiles of the one-step kernel, tiles of the two-step kernel, and
+    // those grown by 3 cells per side (a receiver lies in at most 2 x 2 of them)
+    std::vector<int> is, is2, isx;
+    std::vector<RecRef> il, il2, ilx;
+    std::vector<InjGroup> ig, ig2, igx;
+    bs.injx_cap = 4 * ngmax;
+    TRY(build_injection(pl, bs.bwd, 0, ngmax, is, ig, il));
+    TRY(build_injection(pl, bs.bwd2, 0, ngmax, is2, ig2, il2));
+    TRY(build_injection(pl, bs.bwd2, 3, bs.injx_cap, isx, igx, ilx));
     // strip slots by forward tile
     std::vector<std::pair<int, StripRef>> sv;
     for (int k = 0; k < d.S; ++k) {
@@ -568,6 +621,12 @@ static int batched_init(swit_plan *pl) {
     TRY(pl->alloc(&bs.inj_start, is.size()));
     TRY(pl->alloc(&bs.inj_list, il.size()));
     TRY(pl->alloc(&bs.inj_groups, ig.size()));
+    TRY(pl->alloc(&bs.inj2_start, is2.size()));
+    TRY(pl->alloc(&bs.inj2_list, il2.size()));
+    TRY(pl->alloc(&bs.inj2_groups, ig2.size()));
+    TRY(pl->alloc(&bs.injx_start, isx.size()));
+    TRY(pl->alloc(&bs.injx_list, ilx.size()));
+    TRY(pl->alloc(&bs.injx_groups, igx.size()));
     TRY(pl->alloc(&bs.strip_start, ss.size()));
     TRY(pl->alloc(&bs.strip_list, sl.size()));
     TRY(pl->alloc(&bs.KB, np));
@@ -608,6 +667,12 @@ static int batched_init(swit_plan *pl) {
     HIP_TRY(hipMemcpy(bs.inj_start, is.data(), sizeof(int) * is.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(bs.inj_list, il.data(), sizeof(RecRef) * il.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(bs.inj_groups, ig.data(), sizeof(InjGroup) * ig.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(bs.inj2_start, is2.data(), sizeof(int) * is2.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(bs.inj2_list, il2.data(), sizeof(RecRef) * il2.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(bs.inj2_groups, ig2.data(), sizeof(InjGroup) * ig2.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(bs.injx_start, isx.data(), sizeof(int) * isx.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(bs.injx_list, ilx.data(), sizeof(RecRef) * ilx.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(bs.injx_groups, igx.data(), sizeof(InjGroup) * igx.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(bs.strip_start, ss.data(), sizeof(int) * ss.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(bs.strip_list, sl.data(), sizeof(StripRef) * sl.size(), hipMemcpyHostToDevice));
     // every state buffer starts at zero; cells no step writes stay zero for good
@@ -717,6 +782,32 @@ static void launch_bwd(swit_plan *pl, int it, const float *ri, float *ro, const 
                        bs.inj_groups, bs.inj_list, pl->sadj_t, bs.acc);
 }
 
+template <int SM>
+static void launch_bwd2(swit_plan *pl, int it, const float *ri, float *ro, const float *bi, float *bo,
+                        hipStream_t st) {
+    BatchedState &bs = pl->bat;
+    const Dims &d = pl->d;
+    const void *strips = SM == 0 ? static_cast<const void *>(pl->strips) : static_cast<const void *>(bs.strips16);
+    hipLaunchKernelGGL(k_bwd_fused2<SM>, dim3(pl->ns * bs.bwd2.ntiles), dim3(kThreads), 0, st, d, it, bs.bwd2,
+                       bs.KB, bs.CP, bs.CV, pl->geo, bs.src, strips, bs.smax, ri, ro, bi, bo, bs.inj2_start,
+                       bs.inj2_groups, bs.inj2_list, bs.injx_start, bs.injx_groups, bs.injx_list, bs.injx_cap,
+                       pl->sadj_t, bs.acc);
+}
+
+// Backward launch that takes the reverse pass from step it down: two steps (it, it-1)
+// when the plan runs the two-step kernel and it >= 3, else one.  Returns the steps done.
+static int launch_backward(swit_plan *pl, int it, const float *ri, float *ro, const float *bi, float *bo,
+                           hipStream_t st) {
+    BatchedState &bs = pl->bat;
+    const bool two = bs.bwd_steps == 2 && it >= 3;
+    switch (bs.strip_mode) {
+    case 0: two ? launch_bwd2<0>(pl, it, ri, ro, bi, bo, st) : launch_bwd<0>(pl, it, ri, ro, bi, bo, st); break;
+    case 1: two ? launch_bwd2<1>(pl, it, ri, ro, bi, bo, st) : launch_bwd<1>(pl, it, ri, ro, bi, bo, st); break;
+    default: two ? launch_bwd2<2>(pl, it, ri, ro, bi, bo, st) : launch_bwd<2>(pl, it, ri, ro, bi, bo, st); break;
+    }
+    return two ? 2 : 1;
+}
+
 static int forward_loop(swit_plan *pl, int save, hipStream_t st) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
@@ -770,15 +861,8 @@ static int adjoint_prologue(swit_plan *pl, hipStream_t st) {
 static int adjoint_loop(swit_plan *pl, hipStream_t st) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
-    for (int it = d.nt; it >= 2; --it) {
-        int k = d.nt - it;
-        const float *ri = bs.R[k % 2], *bi = bs.Bk[k % 2];
-        float *ro = bs.R[(k + 1) % 2], *bo = bs.Bk[(k + 1) % 2];
-        switch (bs.strip_mode) {
-        case 0: launch_bwd<0>(pl, it, ri, ro, bi, bo, st); break;
-        case 1: launch_bwd<1>(pl, it, ri, ro, bi, bo, st); break;
-        default: launch_bwd<2>(pl, it, ri, ro, bi, bo, st); break;
-        }
+    for (int it = d.nt, k = 0; it >= 2; k ^= 1) {
+        it -= launch_backward(pl, it, bs.R[k], bs.R[k ^ 1], bs.Bk[k], bs.Bk[k ^ 1], st);
         LAUNCH_CHECK();
     }
     return SWIT_OK;
@@ -871,8 +955,9 @@ static void drop_graphs(BatchedState &bs) {
     ++bs.gen;
 }
 
-// Average duration (us) of `steps` back-to-back step-kernel launches (which = 0 forward
-// with strip saving, 1 backward) on the plan's buffers, after two warm-up launches.
+// Average duration per time step (us) of back-to-back step-kernel launches covering
+// `steps` steps (which = 0 forward with strip saving, 1 the plan's backward kernel) on the
+// plan's buffers, after two warm-up launches.
 static int time_steps(swit_plan *pl, int which, int steps, hipStream_t st, double *us) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
@@ -883,6 +968,9 @@ static int time_steps(swit_plan *pl, int which, int steps, hipStream_t st, doubl
         (void)hipEventDestroy(e0);
         return set_err(SWIT_EHIP, "hipEventCreate: %s", hipGetErrorString(e));
     }
+    // steps per launch and launches timed
+    const int spl = which == 1 && bs.bwd_steps == 2 && d.nt >= 3 ? 2 : 1;
+    const int nl = cdiv(steps, spl);
     auto launch = [&](int k) {
         if (which == 0) {
             const int it = 2 + k % (d.nt - 1);
@@ -892,28 +980,23 @@ static int time_steps(swit_plan *pl, int which, int steps, hipStream_t st, doubl
             default: launch_fwd<2>(pl, it, bs.F[k & 1], bs.F[(k + 1) & 1], 1, st); break;
             }
         } else {
-            const int it = d.nt - k % (d.nt - 1);
-            const float *ri = bs.R[k & 1], *bi = bs.Bk[k & 1];
-            float *ro = bs.R[(k + 1) & 1], *bo = bs.Bk[(k + 1) & 1];
-            switch (bs.strip_mode) {
-            case 0: launch_bwd<0>(pl, it, ri, ro, bi, bo, st); break;
-            case 1: launch_bwd<1>(pl, it, ri, ro, bi, bo, st); break;
-            default: launch_bwd<2>(pl, it, ri, ro, bi, bo, st); break;
-            }
+            // the plan's backward kernel; a two-step launch starts at it >= 3
+            const int it = spl == 2 ? d.nt - (2 * k) % (d.nt - 2) : d.nt - k % (d.nt - 1);
+            launch_backward(pl, it, bs.R[k & 1], bs.R[(k + 1) & 1], bs.Bk[k & 1], bs.Bk[(k + 1) & 1], st);
         }
     };
     int rc = SWIT_OK;
     for (int k = 0; k < 2; ++k) launch(k);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipEventRecord(e0, st);
-    for (int k = 0; k < steps && e == hipSuccess; ++k) launch(k);
+    for (int k = 0; k < nl && e == hipSuccess; ++k) launch(k);
     if (e == hipSuccess) e = hipGetLastError();
     if (e == hipSuccess) e = hipEventRecord(e1, st);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float ms = 0.f;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
     if (e != hipSuccess) rc = set_err(SWIT_EHIP, "autotune timing: %s", hipGetErrorString(e));
-    *us = 1000.0 * ms / steps;
+    *us = 1000.0 * ms / ((double)nl * spl);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return rc;
@@ -952,25 +1035,32 @@ extern "C" int swit_plan_autotune(swit_plan *pl, int steps, int *choice, double 
             if (t < best_f) { best_f = t; bg = G; bcf = cg; }
         }
     }
-    for (int cg : kCols) {
-        if (cg > bs.bwd.nxt) continue;
-        bs.bwd.cg = cg;
-        double t = 1e30;
-        for (int r = 0; r < kRepeats; ++r) {
-            double tr;
-            TRY(time_steps(pl, 1, steps, stream, &tr));
-            t = std::min(t, tr);
+    int bsteps = bs.bwd_steps;
+    for (int sp = 1; sp <= 2; ++sp) {
+        if (sp == 2 && pl->d.nt < 3) continue;
+        bs.bwd_steps = sp;
+        for (int cg : kCols) {
+            if (cg > bs.bwd.nxt) continue;
+            bs.bwd.cg = bs.bwd2.cg = cg;
+            double t = 1e30;
+            for (int r = 0; r < kRepeats; ++r) {
+                double tr;
+                TRY(time_steps(pl, 1, steps, stream, &tr));
+                t = std::min(t, tr);
+            }
+            if (t < best_b) { best_b = t; bcb = cg; bsteps = sp; }
         }
-        if (t < best_b) { best_b = t; bcb = cg; }
     }
     bs.Gf = bg;
     bs.ngroups_f = cdiv(pl->ns, bg);
     bs.fwd.cg = bcf;
-    bs.bwd.cg = bcb;
+    bs.bwd.cg = bs.bwd2.cg = bcb;
+    bs.bwd_steps = bsteps;
     drop_graphs(bs);
     choice[0] = bg;
     choice[1] = bcf;
     choice[2] = bcb;
+    choice[3] = bsteps;
     us[0] = best_f;
     us[1] = best_b;
     return SWIT_OK;
@@ -1003,9 +1093,15 @@ extern "C" int swit_plan_set_option(swit_plan *pl, int option, int value) {
         TileMap &tm = option == SWIT_OPT_FWD_COLGROUP ? bs.fwd : bs.bwd;
         if (value < 1 || value > tm.nxt) return set_err(SWIT_EINVAL, "column group %d outside [1, %d]", value, tm.nxt);
         tm.cg = value;
+        if (option == SWIT_OPT_BWD_COLGROUP) bs.bwd2.cg = value;
         drop_graphs(bs);
         return SWIT_OK;
     }
+    case SWIT_OPT_BWD_STEPS:
+        if (value != 1 && value != 2) return set_err(SWIT_EINVAL, "backward steps per launch must be 1 or 2");
+        bs.bwd_steps = value;
+        drop_graphs(bs);
+        return SWIT_OK;
     default:
         return set_err(SWIT_EINVAL, "unknown option %d", option);
     }

@@ -1,0 +1,102 @@
+"""Two-step backward kernel (two reverse time steps per launch) against the one-step
+kernel and the oracle: identical bits for odd and even step counts, every strip mode,
+free-surface rows at many offsets inside the two-step windows, and receivers on every
+row of columns at and around tile edges (adjoint sources injected in both steps'
+regions, receivers sharing cells added in receiver order)."""
+import numpy as np
+import pytest
+
+from helpers import integrated_sources, rel_l2
+from oracle import oracle as O
+from swit_amd import workloads
+
+
+def _receivers(w, tile=16):
+    """Surface row, plus full-depth lines on interior columns whose padded column lies
+    at or next to a tile edge, plus duplicates of a few cells."""
+    nxp = w.nx + 2 * w.npml
+    cols = []
+    for edge in range(tile, nxp, tile):                  # last padded column of a tile
+        for ix in (edge - 2, edge, edge + 1, edge + 3):
+            jx = ix - w.npml                             # interior column (1-based)
+            if 1 <= jx <= w.nx and jx - 1 not in cols:
+                cols.append(jx - 1)
+    cols = cols[:6]
+    rec = [[x * w.dx, 0.0] for x in range(w.nx)]
+    for c in cols:
+        rec += [[c * w.dx, z * w.dx] for z in range(1, w.nz - 1)]
+    rec += [[cols[0] * w.dx, 2 * w.dx], [cols[0] * w.dx + 1.0, 2 * w.dx + 2.0], [3 * w.dx, 0.0]]
+    rec = np.asarray(rec, np.float32)
+    return np.broadcast_to(rec, (w.ns,) + rec.shape).copy()
+
+
+def _plan(w, strips, steps):
+    import torch
+    from swit_amd.engine import Plan
+    p = Plan(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, w.src_xz, w.rec_xz, strips=strips)
+    p.set_model(torch.from_numpy(w.vp_init), torch.from_numpy(workloads.gardner(w.vp_init)))
+    p.set_option("bwd_steps", steps)
+    return p
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nx,nz,npml,nt,ns,strips", [
+    (60, 50, 10, 161, 2, "f32"),    # odd number of reverse steps: pairs only
+    (60, 50, 10, 160, 2, "f32"),    # even: the pass ends with one one-step launch
+    (40, 30, 59, 120, 2, "f16"),    # wide absorbing layer, free surface deep in the grid
+    (90, 140, 20, 151, 3, "bf16"),  # several two-step tile rows and columns
+    (33, 9, 3, 101, 2, "f32"),      # thinnest absorbing layer and interior
+    (50, 60, 45, 90, 2, "f32"),
+    (50, 60, 50, 91, 2, "f16"),
+])
+def test_two_step_backward_matches_one_step(nx, nz, npml, nt, ns, strips):
+    import torch
+    w = workloads.small(seed=nx + nz + npml, nx=nx, nz=nz, npml=npml, nt=nt, ns=ns)
+    w.rec_xz = _receivers(w)
+    src = integrated_sources(w)
+    m = O.Model(w.vp_true, workloads.gardner(w.vp_true), w.dx, w.npml, "f32")
+    obs = O.forward(m, w.nt, w.dt, w.src_xz, w.rec_xz, src, save_strips=False)["seis"]
+    out = {}
+    for steps in (1, 2):
+        for graph in (1, 0):
+            p = _plan(w, strips, steps)
+            p.set_option("graph", graph)
+            syn = p.forward(torch.from_numpy(src), save_strips=True)
+            adj = (syn.cpu().numpy() - obs).astype(np.float32)
+            out[steps, graph] = [t.cpu().numpy() for t in p.adjoint(torch.from_numpy(src), torch.from_numpy(adj))]
+            p.close()
+    ref = out[1, 1]
+    for key, g in out.items():
+        for x, y in zip(g, ref):
+            np.testing.assert_array_equal(x, y, err_msg=str(key))
+    if strips == "f32":
+        mi = O.Model(w.vp_init, workloads.gardner(w.vp_init), w.dx, w.npml, "f32")
+        f = O.forward(mi, w.nt, w.dt, w.src_xz, w.rec_xz, src)
+        a = O.adjoint(mi, w.nt, w.dt, w.src_xz, w.rec_xz, src, (f["seis"] - obs).astype(np.float32), f)
+        assert rel_l2(ref[0], a["grad"]) <= 1e-12
+        assert rel_l2(ref[1], a["illum_forw"]) <= 1e-12
+        assert rel_l2(ref[2], a["illum_back"]) <= 1e-12
+
+
+@pytest.mark.gpu
+def test_two_step_is_the_default_and_profiles_per_step():
+    """Plans start with the two-step kernel; profiling counts time steps (nt-1 per
+    reverse pass) whichever kernel runs."""
+    import torch
+    w = workloads.small(seed=5, nx=48, nz=36, npml=10, nt=140, ns=2)
+    src = integrated_sources(w)
+    res = {}
+    for steps in (2, 1):
+        p = _plan(w, "f32", steps)
+        p.set_option("profile", 1)
+        syn = p.forward(torch.from_numpy(src), save_strips=True)
+        g = p.adjoint(torch.from_numpy(src), syn * 0.5)[0].cpu().numpy()
+        bm, bn = p.kernel_time("backward")
+        p.close()
+        assert bn == w.nt - 1 and bm > 0
+        res[steps] = g
+    np.testing.assert_array_equal(res[1], res[2])
+    from swit_amd.engine import Plan
+    p = Plan(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, w.src_xz, w.rec_xz)
+    assert p.bwd_steps == 2
+    p.close()
