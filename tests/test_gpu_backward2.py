@@ -48,6 +48,8 @@ def _plan(w, strips, steps):
     (33, 9, 3, 101, 2, "f32"),      # thinnest absorbing layer and interior
     (50, 60, 45, 90, 2, "f32"),
     (50, 60, 50, 91, 2, "f16"),
+    (40, 30, 10, 3, 2, "f32"),      # one two-step launch is the whole reverse pass
+    (40, 30, 10, 2, 2, "bf16"),     # one one-step launch is the whole reverse pass
 ])
 def test_two_step_backward_matches_one_step(nx, nz, npml, nt, ns, strips):
     import torch
