@@ -84,9 +84,20 @@ def autograd_leg(w, src, obs, vp_init, rho_init, args):
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     pe.close()
+    # algorithmic bytes of one autograd gradient: the exact forward (36 N_pad + the
+    # divergence history 4 N_pad + 4 ng per shot and step) and the fused exact adjoint
+    # (adjoint state 24 N_pad, history read 4 N_pad, K-gradient read+write 8 N_pad,
+    # coefficients 12 N_pad, adjoint-source read 4 ng); a checkpointed sweep recomputes
+    # the forward once more
+    n_pad = (w.nx + 2 * w.npml) * (w.nz + 2 * w.npml)
+    fwd_b = 40 * n_pad + 4 * w.ng
+    per_grad = w.ns * (w.nt - 1) * ((2 if mode == "checkpoint" else 1) * fwd_b + 48 * n_pad + 4 * w.ng)
+    gbs = per_grad * args.steps / el / 1e9
     return {"value": round(w.ns * args.steps / el, 3), "unit": "shot-gradients/s",
             "ms_per_step": round(1000 * el / args.steps, 2),
             "history": "full" if mode is True else "checkpoints", "history_gib": round(need / 2**30, 1),
+            "algorithmic_bytes_per_gradient": int(per_grad), "achieved_gbs": round(gbs, 1),
+            "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
             "finite": bool(torch.isfinite(g).all())}
 
 

@@ -124,6 +124,9 @@ const char *swit_last_error(void);
 #define SWIT_OPT_BWD_COLGROUP 16 /* tile columns per group of the backward work order */
 #define SWIT_OPT_BWD_STEPS 32   /* 2 (default): backward kernel doing two time steps per
                                    launch; 1: one step per launch (same results)      */
+#define SWIT_OPT_EXACT_FUSED 64 /* 1 (default): one fused kernel per exact-adjoint step;
+                                   0: the seven separate passes (same results when no
+                                   two receivers share a cell)                        */
 int swit_plan_set_option(swit_plan *plan, int option, int value);
 
 /* Autotuner (no counterpart in the reference): times `steps` back-to-back launches of

@@ -54,6 +54,7 @@ SWIT_OPT_FWD_GROUP = 4
 SWIT_OPT_FWD_COLGROUP = 8
 SWIT_OPT_BWD_COLGROUP = 16
 SWIT_OPT_BWD_STEPS = 32
+SWIT_OPT_EXACT_FUSED = 64
 
 
 class SwitGrid(ctypes.Structure):

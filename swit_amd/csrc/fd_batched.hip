@@ -1113,6 +1113,12 @@ struct BatchedState {
     int nck = 0;                 // checkpoints: states after steps 1, 1+ck, 1+2ck, ...
     float *ckpt = nullptr;       // [nck][ns][3][nxp][ld]
     float *lam = nullptr;        // [ns][3][nxp][ld] adjoint variables of the exact adjoint
+    float *lam2 = nullptr;       // second buffer of the fused exact step (ping-pong with lam)
+    int exact_fused = 1;         // 1: fused exact step kernel; 0: the seven separate passes
+    int *exinj_start = nullptr;  // receiver groups (recording cells) of the backward tiles
+    InjGroup *exinj_groups = nullptr;   // grown by 3 cells, per-shot capacity exinj_cap
+    RecRef *exinj_list = nullptr;
+    int exinj_cap = 0;
     float *qbuf = nullptr;       // [ns][2][nxp][ld] exact-adjoint temporaries
     float *gk = nullptr;         // [ns][nxp][ld] per-shot gradient with respect to K
     float *gseis = nullptr;      // [ns][ngmax][nt] copy of dJ/dseis for the captured loop

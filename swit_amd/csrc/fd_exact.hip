@@ -127,6 +127,206 @@ __global__ void k_ex_vel2(Dims d, const float *__restrict__ q, float *__restrict
     lw[k] = lw[k] - dz;
 }
 
+// ---------------------------------------------------------------------------------
+// One reverse step it of all shots, fused: the operations of k_ex_inject, k_ex_mirror
+// (w), k_ex_vel, k_ex_p, k_ex_mirror (p), k_ex_grad and k_ex_vel2, in that order, with
+// the same arithmetic per cell, so the same bits (receivers sharing a cell are added in
+// receiver order).  One workgroup per tile of the backward tiling (58x16) of one shot.
+// Its window is the tile plus 3 cells per side (64x22): lane l of every wave owns window
+// row l, and a thread's cells are columns wave + 8j of its row.
+// - stage lp, lu, lw of the window; inject into lp; transpose the w mirror;
+// - qu = B lu, qw = B lw on the window (LDS), lu *= fu, lw *= fw (registers);
+// - lp -= Gx^T qu + Gz^T qw on window rows and columns 2 .. extent-2 (the region whose r
+//   the tile's velocity stencils read), then transpose the p mirror (column by column,
+//   in the wave that owns the column);
+// - on the same region: gK -= lp D(it) (tile cells), r = K lp (LDS), lp *= fp;
+// - on the tile: lu -= Dx^T r, lw -= Dz^T r; write lp, lu, lw and gK.
+// The tiling keeps the mirror rows away from window edges where a value they need
+// would lie outside the window: no tile starts at rows fs-1 .. fs+3.
+// LIn/LOut: [ns][3][nxp][ld] (lp, lu, lw), ping-pong; gk: [ns][nxp][ld];
+// g: [ns][ngmax][nt]; inj_*: receiver groups at recording cells of the tiles grown by 3.
+__global__ __launch_bounds__(kThreads) void k_ex_step(
+    Dims d, int it, TileMap tm, const float2 *__restrict__ KB, const float2 *__restrict__ CP,
+    const float4 *__restrict__ CV, DSave D, const float *__restrict__ LIn, float *__restrict__ LOut,
+    float *__restrict__ gk, const int *__restrict__ inj_gstart, const InjGroup *__restrict__ inj_groups,
+    const RecRef *__restrict__ inj_list, int inj_cap, const float *__restrict__ g) {
+    using namespace bk;
+    __shared__ float sP[kNL];
+    __shared__ float sU[kNL];
+    __shared__ float sW[kNL];
+    __shared__ float sA[kNL];     // qu, then r
+    __shared__ float sB[kNL];     // qw
+
+    int s, t;
+    work_item(xcd_logical(blockIdx.x, gridDim.x), (int)(gridDim.x / tm.ntiles), tm, s, t);
+    const TileBox tb = tile_box(d, tm, t);
+    const int zo = tb.z0 - kHalo, xo = tb.x0 - kHalo;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int np = d.nxp * d.ld;
+    const int z = zo + lane;
+    const bool zin = z >= 1 && z <= d.nzp;
+    const bool lane_r = lane >= 2 && lane <= kZL - 2;          // rows of the lp / r region
+    const bool lane_t = lane >= kHalo && z <= tb.z1;           // tile rows
+    const int ct_end = tb.x1 - xo;                             // last tile column
+    const int m = d.fs - 1 - zo;                               // window row of fs-1
+    constexpr int kJS = (kCX + kWaves - 1) / kWaves;
+    const float *lin = LIn + (size_t)s * 3 * np;
+    float *lout = LOut + (size_t)s * 3 * np;
+    float *gks = gk + (size_t)s * np;
+    const float *Dp = D.p + ((size_t)s * D.slots + (it - 1 - D.base)) * np;
+
+    // ---- stage lp, lu, lw ----
+#pragma unroll
+    for (int j = 0; j < kJS; ++j) {
+        const int c = wave + kWaves * j;
+        if (c >= kCX) continue;
+        const int x = xo + c;
+        if (zin && x >= 1 && x <= d.nxp) {
+            const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
+            __builtin_amdgcn_global_load_lds(lin + k, sP + LW(c, 0), 4, 0, 0);
+            __builtin_amdgcn_global_load_lds(lin + np + k, sU + LW(c, 0), 4, 0, 0);
+            __builtin_amdgcn_global_load_lds(lin + 2 * np + k, sW + LW(c, 0), 4, 0, 0);
+        } else {
+            const int l = LW(c, lane);
+            sP[l] = sU[l] = sW[l] = 0.f;
+        }
+    }
+    // coefficients of this thread's cells: B (window), K and 1-kappa_p (region), the
+    // velocity factors, history and gK (tile)
+    float2 kb[kJS];
+    float fu[kJS], fw[kJS], fp[kJS], dv[kJS], gkv[kJS];
+#pragma unroll
+    for (int j = 0; j < kJS; ++j) {
+        const int c = wave + kWaves * j;
+        const int x = xo + c;
+        kb[j] = make_float2(0.f, 0.f);
+        fu[j] = fw[j] = fp[j] = 1.0f;
+        dv[j] = gkv[j] = 0.f;
+        if (c >= kCX || !zin || x < 1 || x > d.nxp) continue;
+        const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
+        kb[j] = KB[k];
+        if (c >= 2 && c <= kCX - 2 && lane_r) fp[j] = CP[k].y;
+        if (c >= kHalo && c <= ct_end && lane_t) {
+            const float4 cv = CV[k];
+            fu[j] = cv.y;
+            fw[j] = cv.z;
+            if (in_rp(d, z, x)) {
+                dv[j] = Dp[k];
+                gkv[j] = gks[k];
+            }
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+
+    // ---- lp(receivers) += g(it) (k_ex_inject), one thread per cell in receiver order ----
+    {
+        const int g0 = inj_gstart[(size_t)s * (tm.ntiles + 1) + t];
+        const int g1 = inj_gstart[(size_t)s * (tm.ntiles + 1) + t + 1];
+        const RecRef *el = inj_list + (size_t)s * inj_cap;
+        for (int q = g0 + (int)threadIdx.x; q < g1; q += kThreads) {
+            const InjGroup gr = inj_groups[(size_t)s * inj_cap + q];
+            const RecRef r0 = el[gr.e0];
+            const int l = LW(r0.ix - xo, r0.iz - zo);
+            float v = sP[l];
+            for (int e = gr.e0; e < gr.e1; ++e) v = v + g[((size_t)s * d.ngmax + el[e].idx) * d.nt + (it - 1)];
+            sP[l] = v;
+        }
+    }
+    // ---- w mirror transposed (k_ex_mirror, w): lw(fs) += lw(fs-1), lw(fs-1) = 0; each
+    // column in the wave that owns it ----
+    if (d.free_surface == 1 && m >= 0 && m < kZL) {
+#pragma unroll
+        for (int j = 0; j < kJS; ++j) {
+            const int c = wave + kWaves * j;
+            const int x = xo + c;
+            if (c >= kCX || x < 1 || x > d.nxp) continue;
+            const float wm = sW[LW(c, m)];
+            if (lane == m + 1) sW[LW(c, m + 1)] = sW[LW(c, m + 1)] + wm;
+            if (lane == m) sW[LW(c, m)] = 0.0f;
+        }
+    }
+    // ---- qu = B lu, qw = B lw (zero off R_u / R_w), lu *= fu, lw *= fw (k_ex_vel) ----
+    float lu[kJS], lw[kJS];
+#pragma unroll
+    for (int j = 0; j < kJS; ++j) {
+        const int c = wave + kWaves * j;
+        lu[j] = lw[j] = 0.f;
+        if (c >= kCX) continue;
+        const int x = xo + c;
+        const int l = LW(c, lane);
+        const bool g_in = zin && x >= 1 && x <= d.nxp;
+        const bool ru = g_in && in_ru(d, z, x), rw = g_in && in_rw(d, z, x);
+        const float u = sU[l], w = sW[l];
+        sA[l] = ru ? kb[j].y * u : 0.0f;
+        sB[l] = rw ? kb[j].y * w : 0.0f;
+        lu[j] = ru ? fu[j] * u : u;
+        lw[j] = rw ? fw[j] * w : w;
+    }
+    __syncthreads();
+    // ---- lp -= Gx^T qu + Gz^T qw (k_ex_p) on the region, then the p mirror transposed
+    // (k_ex_mirror, p): lp(fs+1) -= lp(fs-1), lp(fs-1) = 0 ----
+#pragma unroll
+    for (int j = 0; j < kJS; ++j) {
+        const int c = wave + kWaves * j;
+        const int x = xo + c;
+        if (c < 2 || c > kCX - 2 || !lane_r || !zin || x < 1 || x > d.nxp) continue;
+        const int l = LW(c, lane);
+        const float gx = kC1 * (sA[l - kZL] - sA[l]) + kC2 * (sA[l - 2 * kZL] - sA[l + kZL]);
+        const float gz = kC1 * (sB[l - 1] - sB[l]) + kC2 * (sB[l - 2] - sB[l + 1]);
+        sP[l] = sP[l] - gx - gz;
+    }
+    if (d.free_surface == 1 && m >= 0 && m < kZL) {
+#pragma unroll
+        for (int j = 0; j < kJS; ++j) {
+            const int c = wave + kWaves * j;
+            const int x = xo + c;
+            if (c < 2 || c > kCX - 2 || x < 1 || x > d.nxp) continue;
+            const float pm = sP[LW(c, m)];
+            if (m + 2 < kZL && lane == m + 2) sP[LW(c, m + 2)] = sP[LW(c, m + 2)] - pm;
+            if (lane == m) sP[LW(c, m)] = 0.0f;
+        }
+    }
+    __syncthreads();
+    // ---- on R_p: gK -= lp D(it), r = K lp, lp *= fp; r = 0 elsewhere (k_ex_grad) ----
+    float lp[kJS];
+#pragma unroll
+    for (int j = 0; j < kJS; ++j) {
+        const int c = wave + kWaves * j;
+        lp[j] = 0.f;
+        if (c >= kCX) continue;
+        const int x = xo + c;
+        const int l = LW(c, lane);
+        const float p = sP[l];
+        lp[j] = p;
+        if (c < 2 || c > kCX - 2 || !lane_r) continue;
+        float r = 0.0f;
+        if (zin && x >= 1 && x <= d.nxp && in_rp(d, z, x)) {
+            if (c >= kHalo && c <= ct_end && lane_t) gkv[j] = gkv[j] - p * dv[j];
+            r = kb[j].x * p;
+            lp[j] = fp[j] * p;
+        }
+        sA[l] = r;
+    }
+    __syncthreads();
+    // ---- lu -= Dx^T r, lw -= Dz^T r (k_ex_vel2) on the tile; write back ----
+#pragma unroll
+    for (int j = 0; j < kJS; ++j) {
+        const int c = wave + kWaves * j;
+        if (c < kHalo || c > ct_end || !lane_t) continue;
+        const int x = xo + c;
+        const int l = LW(c, lane);
+        const float dx = kC1 * (sA[l] - sA[l + kZL]) + kC2 * (sA[l - kZL] - sA[l + 2 * kZL]);
+        const float dz = kC1 * (sA[l] - sA[l + 1]) + kC2 * (sA[l - 1] - sA[l + 2]);
+        const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
+        lout[k] = lp[j];
+        lout[np + k] = lu[j] - dx;
+        lout[2 * np + k] = lw[j] - dz;
+        if (in_rp(d, z, x)) gks[k] = gkv[j];
+    }
+}
+
 // dJ/dc on the interior: sum over shots (FP64) of the K-gradient of every padded cell
 // that replicates the interior cell (io.f90:231-300 padding), times dK/dc = 2 den c dt/dx.
 __global__ void k_ex_fold(Dims d, int ns, const float *__restrict__ vp, const float *__restrict__ rho,

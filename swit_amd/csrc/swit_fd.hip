@@ -447,7 +447,10 @@ static int tile_of(const TileMap &tm, int iz, int ix) {
 // Backward tiling: rows 2..nzp-1 are the only rows a backward step changes (adjoint
 // u/w rows 2..nzp-1, p rows 3..nzp-2; the interior window lies inside), and rows
 // fs-2..fs+1 must share a tile (the adjoint p stencil there reads the mirrored w(fs-1);
-// the p mirror reads row fs+1 after injection).  Pick the offset with the fewest tiles.
+// the p mirror reads row fs+1 after injection), so no tile starts at fs-1..fs+1.  The
+// fused exact-adjoint step uses the same tiles and needs, in every tile's window (tile
+// +- 3 rows), the rows its transposed mirrors read: no tile starts at fs+2 or fs+3
+// either.  Pick the offset with the fewest tiles.
 static TileMap backward_tilemap(const Dims &d) {
     const int th = bk::kTZ, fs = d.fs;
     TileMap best{};
@@ -456,7 +459,7 @@ static TileMap backward_tilemap(const Dims &d) {
         int ztop = 2 - shift;
         bool ok = true;
         for (int b = ztop + th; b <= d.nzp; b += th)       // tile starts after the first
-            if (b >= fs - 1 && b <= fs + 1) ok = false;
+            if (b >= fs - 1 && b <= fs + 3) ok = false;
         if (!ok) continue;
         TileMap tm;
         tm.ztop = ztop;
@@ -497,13 +500,13 @@ static TileMap backward2_tilemap(const Dims &d) {
     return best;
 }
 
-// Injection groups per tile: for every shot, the receivers whose injection cell lies in
-// the tile grown by `margin` cells per side, listed tile by tile, sorted by cell within
+// Injection groups per tile: for every shot, the receivers whose cell (`cells`: padded
+// offsets [ns][ngmax]) lies in the tile grown by `margin` cells per side, listed tile by tile, sorted by cell within
 // a tile (stably, so receivers sharing a cell stay in receiver order) and grouped by
 // cell, so one thread adds every receiver of a cell in order.
 // gst: [ns][ntiles+1] group ranges per tile; groups, list: [ns][cap].
-static int build_injection(swit_plan *pl, const TileMap &tm, int margin, int cap, std::vector<int> &gst,
-                           std::vector<InjGroup> &groups, std::vector<RecRef> &list) {
+static int build_injection(swit_plan *pl, const std::vector<int> &cells, const TileMap &tm, int margin, int cap,
+                           std::vector<int> &gst, std::vector<InjGroup> &groups, std::vector<RecRef> &list) {
     const Dims &d = pl->d;
     const int ns = pl->ns, ngmax = d.ngmax;
     gst.assign((size_t)ns * (tm.ntiles + 1), 0);
@@ -512,7 +515,7 @@ static int build_injection(swit_plan *pl, const TileMap &tm, int margin, int cap
     for (int s = 0; s < ns; ++s) {
         std::vector<RecRef> rec;
         for (int r = 0; r < pl->geo_h[s].ng; ++r) {
-            const int ci = pl->rec_cells_inj[(size_t)s * ngmax + r];
+            const int ci = cells[(size_t)s * ngmax + r];
             rec.push_back(RecRef{r, ci % d.ld + 1, ci / d.ld + 1, 0});
         }
         RecRef *el = &list[(size_t)s * cap];
@@ -593,9 +596,9 @@ static int batched_init(swit_plan *pl) {
     std::vector<RecRef> il, il2, ilx;
     std::vector<InjGroup> ig, ig2, igx;
     bs.injx_cap = 4 * ngmax;
-    TRY(build_injection(pl, bs.bwd, 0, ngmax, is, ig, il));
-    TRY(build_injection(pl, bs.bwd2, 0, ngmax, is2, ig2, il2));
-    TRY(build_injection(pl, bs.bwd2, 3, bs.injx_cap, isx, igx, ilx));
+    TRY(build_injection(pl, pl->rec_cells_inj, bs.bwd, 0, ngmax, is, ig, il));
+    TRY(build_injection(pl, pl->rec_cells_inj, bs.bwd2, 0, ngmax, is2, ig2, il2));
+    TRY(build_injection(pl, pl->rec_cells_inj, bs.bwd2, 3, bs.injx_cap, isx, igx, ilx));
     // strip slots by forward tile
     std::vector<std::pair<int, StripRef>> sv;
     for (int k = 0; k < d.S; ++k) {
@@ -652,8 +655,20 @@ static int batched_init(swit_plan *pl) {
             TRY(pl->alloc(&bs.dhist, (size_t)ns * d.nt * np));
         }
         TRY(pl->alloc(&bs.lam, (size_t)ns * 3 * np));
+        TRY(pl->alloc(&bs.lam2, (size_t)ns * 3 * np));
         TRY(pl->alloc(&bs.qbuf, (size_t)ns * 2 * np));
         TRY(pl->alloc(&bs.gk, (size_t)ns * np));
+        std::vector<int> es;
+        std::vector<RecRef> el;
+        std::vector<InjGroup> eg;
+        bs.exinj_cap = 4 * ngmax;
+        TRY(build_injection(pl, pl->rec_cells_fwd, bs.bwd, 3, bs.exinj_cap, es, eg, el));
+        TRY(pl->alloc(&bs.exinj_start, es.size()));
+        TRY(pl->alloc(&bs.exinj_groups, eg.size()));
+        TRY(pl->alloc(&bs.exinj_list, el.size()));
+        HIP_TRY(hipMemcpy(bs.exinj_start, es.data(), sizeof(int) * es.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(bs.exinj_groups, eg.data(), sizeof(InjGroup) * eg.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(bs.exinj_list, el.data(), sizeof(RecRef) * el.size(), hipMemcpyHostToDevice));
         TRY(pl->alloc(&bs.gseis, (size_t)ns * ngmax * d.nt));
     }
     bs.strip_mode = (pl->flags & SWIT_STRIPS_F16) ? 1 : (pl->flags & SWIT_STRIPS_BF16) ? 2 : 0;
@@ -1097,6 +1112,10 @@ extern "C" int swit_plan_set_option(swit_plan *pl, int option, int value) {
         drop_graphs(bs);
         return SWIT_OK;
     }
+    case SWIT_OPT_EXACT_FUSED:
+        bs.exact_fused = value != 0;
+        if (bs.gex_b) { (void)hipGraphExecDestroy(bs.gex_b); bs.gex_b = nullptr; }
+        return SWIT_OK;
     case SWIT_OPT_BWD_STEPS:
         if (value != 1 && value != 2) return set_err(SWIT_EINVAL, "backward steps per launch must be 1 or 2");
         bs.bwd_steps = value;
@@ -1366,10 +1385,22 @@ static int exact_forward_loop(swit_plan *pl, hipStream_t st) {
 }
 
 // Reverse steps b..a+1 of the exact adjoint, reading dJ/dseis from bs.gseis.
-static int exact_adjoint_steps(swit_plan *pl, int a, int b, DSave ds, hipStream_t st) {
+// Reverse steps b..a+1 of the exact adjoint.  The fused kernel reads lam[k] and writes
+// lam[k^1] (lam[0] = bs.lam, lam[1] = bs.lam2); k carries over between segments.
+static int exact_adjoint_steps(swit_plan *pl, int a, int b, DSave ds, int &k, hipStream_t st) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
     const int ns = pl->ns;
+    if (bs.exact_fused) {
+        float *lam[2] = {bs.lam, bs.lam2};
+        for (int it = b; it >= a + 1; --it, k ^= 1) {
+            hipLaunchKernelGGL(ex::k_ex_step, dim3(ns * bs.bwd.ntiles), dim3(kThreads), 0, st, d, it, bs.bwd, bs.KB,
+                               bs.CP, bs.CV, ds, lam[k], lam[k ^ 1], bs.gk, bs.exinj_start, bs.exinj_groups,
+                               bs.exinj_list, bs.exinj_cap, bs.gseis);
+            LAUNCH_CHECK();
+        }
+        return SWIT_OK;
+    }
     const dim3 blk(64, 4), grd(cdiv(d.nzp, 64), cdiv(d.nxp, 4), ns);
     const dim3 gcol(cdiv(d.nxp, 256), ns), ginj(cdiv(d.ngmax, 256), ns);
     for (int it = b; it >= a + 1; --it) {
@@ -1390,13 +1421,14 @@ static int exact_adjoint_steps(swit_plan *pl, int a, int b, DSave ds, hipStream_
 static int exact_adjoint_loop(swit_plan *pl, hipStream_t st) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
-    if (!bs.ck) return exact_adjoint_steps(pl, 1, d.nt, DSave{bs.dhist, d.nt, 0}, st);
+    int k = 0;                      // the sweep starts from lam[0] (both buffers are zero)
+    if (!bs.ck) return exact_adjoint_steps(pl, 1, d.nt, DSave{bs.dhist, d.nt, 0}, k, st);
     const size_t chunk = (size_t)pl->ns * 3 * d.nxp * d.ld;
     for (int j = bs.nck - 1; j >= 0; --j) {
         const int a = 1 + j * bs.ck, b = std::min(a + bs.ck, d.nt);
         TRY(copyk(bs.ckpt + (size_t)j * chunk, bs.F[(a + 1) % 2], chunk, st));
         TRY(exact_forward_steps(pl, a, b, DSave{bs.dhist, bs.ck, a}, st));
-        TRY(exact_adjoint_steps(pl, a, b, DSave{bs.dhist, bs.ck, a}, st));
+        TRY(exact_adjoint_steps(pl, a, b, DSave{bs.dhist, bs.ck, a}, k, st));
     }
     return SWIT_OK;
 }
@@ -1454,6 +1486,7 @@ extern "C" int swit_adjoint_exact(swit_plan *pl, const float *gseis, double *gra
     const size_t np = (size_t)d.nxp * d.ld;
     HIP_TRY(hipMemcpyAsync(bs.gseis, gseis, sizeof(float) * ns * d.ngmax * d.nt, hipMemcpyDeviceToDevice, stream));
     TRY(fill(bs.lam, 3 * np * ns, stream));
+    TRY(fill(bs.lam2, 3 * np * ns, stream));
     TRY(fill(bs.gk, np * ns, stream));
     if (bs.use_graph) {
         if (!bs.gex_b) TRY(exact_capture(pl, 1, &bs.gex_b));

@@ -97,10 +97,11 @@ class Plan:
 
     def set_option(self, option, value):
         """option: 'graph' | 'profile' | 'fwd_group' | 'fwd_colgroup' | 'bwd_colgroup' |
-        'bwd_steps' (batched path)."""
+        'bwd_steps' | 'exact_fused' (batched path)."""
         code = {"graph": _native.SWIT_OPT_GRAPH, "profile": _native.SWIT_OPT_PROFILE,
                 "fwd_group": _native.SWIT_OPT_FWD_GROUP, "fwd_colgroup": _native.SWIT_OPT_FWD_COLGROUP,
-                "bwd_colgroup": _native.SWIT_OPT_BWD_COLGROUP, "bwd_steps": _native.SWIT_OPT_BWD_STEPS}[option]
+                "bwd_colgroup": _native.SWIT_OPT_BWD_COLGROUP, "bwd_steps": _native.SWIT_OPT_BWD_STEPS,
+                "exact_fused": _native.SWIT_OPT_EXACT_FUSED}[option]
         _native.check(self._lib.swit_plan_set_option(self._h, code, int(value)))
         if option == "bwd_steps":
             self.bwd_steps = int(value)
