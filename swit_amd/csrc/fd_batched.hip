@@ -1121,6 +1121,7 @@ struct BatchedState {
     int exinj_cap = 0;
     float *qbuf = nullptr;       // [ns][2][nxp][ld] exact-adjoint temporaries
     float *gk = nullptr;         // [ns][nxp][ld] per-shot gradient with respect to K
+    double *gsum = nullptr;      // [nxp][ld] K-gradient summed over shots (FP64)
     float *gseis = nullptr;      // [ns][ngmax][nt] copy of dJ/dseis for the captured loop
     hipGraphExec_t gex_f = nullptr, gex_b = nullptr;   // exact forward / adjoint loops
     bool exact_valid = false;    // dhist holds the last forward_exact of the current model

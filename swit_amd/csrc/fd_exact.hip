@@ -327,23 +327,31 @@ __global__ __launch_bounds__(kThreads) void k_ex_step(
     }
 }
 
-// dJ/dc on the interior: sum over shots (FP64) of the K-gradient of every padded cell
-// that replicates the interior cell (io.f90:231-300 padding), times dK/dc = 2 den c dt/dx.
-__global__ void k_ex_fold(Dims d, int ns, const float *__restrict__ vp, const float *__restrict__ rho,
-                          const float *__restrict__ gk, double *__restrict__ grad) {
+// FP64 sum over shots (in shot order) of the K-gradient of every padded cell.
+// gs: [nxp][ld].  grid-stride over the cells.
+__global__ void k_ex_shotsum(Dims d, int ns, const float *__restrict__ gk, double *__restrict__ gs) {
+    const size_t np = (size_t)d.nxp * d.ld;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < np; k += (size_t)gridDim.x * blockDim.x) {
+        double sum = 0.0;
+        for (int s = 0; s < ns; ++s) sum += (double)gk[(size_t)s * np + k];
+        gs[k] = sum;
+    }
+}
+
+// dJ/dc on the interior: the shot-summed K-gradient of every padded cell that replicates
+// the interior cell (io.f90:231-300 padding), summed in FP64 column by column, times
+// dK/dc = 2 den c dt/dx.
+__global__ void k_ex_fold(Dims d, const float *__restrict__ vp, const float *__restrict__ rho,
+                          const double *__restrict__ gs, double *__restrict__ grad) {
     const int jz = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int jx = blockIdx.y * blockDim.y + threadIdx.y + 1;
     if (jz > d.nz || jx > d.nx) return;
     const int npml = d.npml;
     const int z0 = jz == 1 ? 1 : npml + jz, z1 = jz == d.nz ? d.nzp : npml + jz;
     const int x0 = jx == 1 ? 1 : npml + jx, x1 = jx == d.nx ? d.nxp : npml + jx;
-    const size_t np = (size_t)d.nxp * d.ld;
     double sum = 0.0;
-    for (int s = 0; s < ns; ++s) {
-        const float *g = gk + (size_t)s * np;
-        for (int ix = x0; ix <= x1; ++ix)
-            for (int iz = z0; iz <= z1; ++iz) sum += (double)g[pidx(d, iz, ix)];
-    }
+    for (int ix = x0; ix <= x1; ++ix)
+        for (int iz = z0; iz <= z1; ++iz) sum += gs[pidx(d, iz, ix)];
     const size_t ki = iidx(d, jz, jx);
     grad[ki] = sum * 2.0 * (double)rho[ki] * (double)vp[ki] * (double)d.dt / (double)d.dx;
 }

@@ -658,6 +658,7 @@ static int batched_init(swit_plan *pl) {
         TRY(pl->alloc(&bs.lam2, (size_t)ns * 3 * np));
         TRY(pl->alloc(&bs.qbuf, (size_t)ns * 2 * np));
         TRY(pl->alloc(&bs.gk, (size_t)ns * np));
+        TRY(pl->alloc(&bs.gsum, np));
         std::vector<int> es;
         std::vector<RecRef> el;
         std::vector<InjGroup> eg;
@@ -1494,8 +1495,11 @@ extern "C" int swit_adjoint_exact(swit_plan *pl, const float *gseis, double *gra
     } else {
         TRY(exact_adjoint_loop(pl, stream));
     }
-    hipLaunchKernelGGL(ex::k_ex_fold, dim3(cdiv(d.nz, 64), cdiv(d.nx, 4)), dim3(64, 4), 0, stream, d, ns, pl->c_int,
-                       pl->rho_int, bs.gk, grad);
+    hipLaunchKernelGGL(ex::k_ex_shotsum, dim3((unsigned)std::min<size_t>(cdiv((int)np, 256), 4096)), dim3(256), 0,
+                       stream, d, ns, bs.gk, bs.gsum);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(ex::k_ex_fold, dim3(cdiv(d.nz, 64), cdiv(d.nx, 4)), dim3(64, 4), 0, stream, d, pl->c_int,
+                       pl->rho_int, bs.gsum, grad);
     LAUNCH_CHECK();
     return SWIT_OK;
 }
