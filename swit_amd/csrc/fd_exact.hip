@@ -133,9 +133,13 @@ __global__ void k_ex_vel2(Dims d, const float *__restrict__ q, float *__restrict
 // the same arithmetic per cell, so the same bits (receivers sharing a cell are added in
 // receiver order).  One workgroup per tile of the backward tiling (58x16) of one shot.
 // Its window is the tile plus 3 cells per side (64x22): lane l of every wave owns window
-// row l, and a thread's cells are columns wave + 8j of its row.
-// - stage lp, lu, lw of the window; inject into lp; transpose the w mirror;
-// - qu = B lu, qw = B lw on the window (LDS), lu *= fu, lw *= fw (registers);
+// row l, and a thread's cells are columns wave + 8j of its row, so the z neighbours of a
+// cell are the adjacent lanes' values (qw, r and the w mirror use DPP wave shifts;
+// lane_up: lane l gets lane l+1's value, lane_dn: lane l-1's, 0 past the ends, called
+// with every lane active).
+// - stage lp of the window in LDS and lu, lw of the thread's cells in registers (only
+//   their own cells read them); inject into lp; transpose the w mirror;
+// - qu = B lu (LDS), qw = B lw (registers), lu *= fu, lw *= fw on the window;
 // - lp -= Gx^T qu + Gz^T qw on window rows and columns 2 .. extent-2 (the region whose r
 //   the tile's velocity stencils read), then transpose the p mirror (column by column,
 //   in the wave that owns the column);
@@ -145,6 +149,12 @@ __global__ void k_ex_vel2(Dims d, const float *__restrict__ q, float *__restrict
 // would lie outside the window: no tile starts at rows fs-1 .. fs+3.
 // LIn/LOut: [ns][3][nxp][ld] (lp, lu, lw), ping-pong; gk: [ns][nxp][ld];
 // g: [ns][ngmax][nt]; inj_*: receiver groups at recording cells of the tiles grown by 3.
+__device__ __forceinline__ float lane_up(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float lane_dn(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
+}
 __global__ __launch_bounds__(kThreads) void k_ex_step(
     Dims d, int it, TileMap tm, const float2 *__restrict__ KB, const float2 *__restrict__ CP,
     const float4 *__restrict__ CV, DSave D, const float *__restrict__ LIn, float *__restrict__ LOut,
@@ -152,10 +162,7 @@ __global__ __launch_bounds__(kThreads) void k_ex_step(
     const RecRef *__restrict__ inj_list, int inj_cap, const float *__restrict__ g) {
     using namespace bk;
     __shared__ float sP[kNL];
-    __shared__ float sU[kNL];
-    __shared__ float sW[kNL];
     __shared__ float sA[kNL];     // qu, then r
-    __shared__ float sB[kNL];     // qw
 
     int s, t;
     work_item(xcd_logical(blockIdx.x, gridDim.x), (int)(gridDim.x / tm.ntiles), tm, s, t);
@@ -176,20 +183,22 @@ __global__ __launch_bounds__(kThreads) void k_ex_step(
     float *gks = gk + (size_t)s * np;
     const float *Dp = D.p + ((size_t)s * D.slots + (it - 1 - D.base)) * np;
 
-    // ---- stage lp, lu, lw ----
+    // ---- stage lp (LDS: injection writes other threads' cells); lu, lw of this
+    // thread's cells into registers (only their own cells read them) ----
+    float lu[kJS], lw[kJS], qw[kJS];
 #pragma unroll
     for (int j = 0; j < kJS; ++j) {
         const int c = wave + kWaves * j;
+        lu[j] = lw[j] = qw[j] = 0.f;
         if (c >= kCX) continue;
         const int x = xo + c;
         if (zin && x >= 1 && x <= d.nxp) {
             const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
             __builtin_amdgcn_global_load_lds(lin + k, sP + LW(c, 0), 4, 0, 0);
-            __builtin_amdgcn_global_load_lds(lin + np + k, sU + LW(c, 0), 4, 0, 0);
-            __builtin_amdgcn_global_load_lds(lin + 2 * np + k, sW + LW(c, 0), 4, 0, 0);
+            lu[j] = lin[np + k];
+            lw[j] = lin[2 * np + k];
         } else {
-            const int l = LW(c, lane);
-            sP[l] = sU[l] = sW[l] = 0.f;
+            sP[LW(c, lane)] = 0.f;
         }
     }
     // coefficients of this thread's cells: B (window), K and 1-kappa_p (region), the
@@ -242,25 +251,23 @@ __global__ __launch_bounds__(kThreads) void k_ex_step(
             const int c = wave + kWaves * j;
             const int x = xo + c;
             if (c >= kCX || x < 1 || x > d.nxp) continue;
-            const float wm = sW[LW(c, m)];
-            if (lane == m + 1) sW[LW(c, m + 1)] = sW[LW(c, m + 1)] + wm;
-            if (lane == m) sW[LW(c, m)] = 0.0f;
+            const float wm = lane_dn(lw[j]);            // lane m+1 gets lane m's lw
+            if (lane == m + 1) lw[j] = lw[j] + wm;
+            if (lane == m) lw[j] = 0.0f;
         }
     }
     // ---- qu = B lu, qw = B lw (zero off R_u / R_w), lu *= fu, lw *= fw (k_ex_vel) ----
-    float lu[kJS], lw[kJS];
 #pragma unroll
     for (int j = 0; j < kJS; ++j) {
         const int c = wave + kWaves * j;
-        lu[j] = lw[j] = 0.f;
         if (c >= kCX) continue;
         const int x = xo + c;
         const int l = LW(c, lane);
         const bool g_in = zin && x >= 1 && x <= d.nxp;
         const bool ru = g_in && in_ru(d, z, x), rw = g_in && in_rw(d, z, x);
-        const float u = sU[l], w = sW[l];
+        const float u = lu[j], w = lw[j];
         sA[l] = ru ? kb[j].y * u : 0.0f;
-        sB[l] = rw ? kb[j].y * w : 0.0f;
+        qw[j] = rw ? kb[j].y * w : 0.0f;
         lu[j] = ru ? fu[j] * u : u;
         lw[j] = rw ? fw[j] * w : w;
     }
@@ -271,10 +278,13 @@ __global__ __launch_bounds__(kThreads) void k_ex_step(
     for (int j = 0; j < kJS; ++j) {
         const int c = wave + kWaves * j;
         const int x = xo + c;
-        if (c < 2 || c > kCX - 2 || !lane_r || !zin || x < 1 || x > d.nxp) continue;
+        if (c < 2 || c > kCX - 2 || x < 1 || x > d.nxp) continue;
+        // qw of the rows above and below: the neighbouring lanes' values
+        const float qd1 = lane_dn(qw[j]), qd2 = lane_dn(qd1), qu1 = lane_up(qw[j]);
+        if (!lane_r || !zin) continue;
         const int l = LW(c, lane);
         const float gx = kC1 * (sA[l - kZL] - sA[l]) + kC2 * (sA[l - 2 * kZL] - sA[l + kZL]);
-        const float gz = kC1 * (sB[l - 1] - sB[l]) + kC2 * (sB[l - 2] - sB[l + 1]);
+        const float gz = kC1 * (qd1 - qw[j]) + kC2 * (qd2 - qu1);
         sP[l] = sP[l] - gx - gz;
     }
     if (d.free_surface == 1 && m >= 0 && m < kZL) {
@@ -290,11 +300,11 @@ __global__ __launch_bounds__(kThreads) void k_ex_step(
     }
     __syncthreads();
     // ---- on R_p: gK -= lp D(it), r = K lp, lp *= fp; r = 0 elsewhere (k_ex_grad) ----
-    float lp[kJS];
+    float lp[kJS], rr[kJS];
 #pragma unroll
     for (int j = 0; j < kJS; ++j) {
         const int c = wave + kWaves * j;
-        lp[j] = 0.f;
+        lp[j] = rr[j] = 0.f;
         if (c >= kCX) continue;
         const int x = xo + c;
         const int l = LW(c, lane);
@@ -308,17 +318,20 @@ __global__ __launch_bounds__(kThreads) void k_ex_step(
             lp[j] = fp[j] * p;
         }
         sA[l] = r;
+        rr[j] = r;
     }
     __syncthreads();
     // ---- lu -= Dx^T r, lw -= Dz^T r (k_ex_vel2) on the tile; write back ----
 #pragma unroll
     for (int j = 0; j < kJS; ++j) {
         const int c = wave + kWaves * j;
-        if (c < kHalo || c > ct_end || !lane_t) continue;
+        if (c < kHalo || c > ct_end) continue;
+        const float ru1 = lane_up(rr[j]), ru2 = lane_up(ru1), rd1 = lane_dn(rr[j]);
+        if (!lane_t) continue;
         const int x = xo + c;
         const int l = LW(c, lane);
         const float dx = kC1 * (sA[l] - sA[l + kZL]) + kC2 * (sA[l - kZL] - sA[l + 2 * kZL]);
-        const float dz = kC1 * (sA[l] - sA[l + 1]) + kC2 * (sA[l - 1] - sA[l + 2]);
+        const float dz = kC1 * (rr[j] - ru1) + kC2 * (rd1 - ru2);
         const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
         lout[k] = lp[j];
         lout[np + k] = lu[j] - dx;
