@@ -203,6 +203,30 @@ int swit_group_destroy(swit_group *group);
 /* Bytes of device scratch the plan holds (wavefields, strips, accumulators). */
 int64_t swit_plan_scratch_bytes(const swit_plan *plan);
 
+/* ---- gradient preconditioning (toolbox/postprocess.py:20-111) --------------------- */
+
+typedef struct swit_precond {
+    int nx, nz;
+    int marine;         /* 1: 'Marine'/'Offshore' (taper zeroes rows above grad_mute, smoothing
+                           below it); 0: land (smoothed Hamming damping, smoothing everywhere) */
+    int grad_mute;      /* optim.grad_mute: taper size in rows (0: no taper) */
+    float grad_thred;   /* optim.grad_thred (land taper floor) */
+    int grad_smooth;    /* optim.grad_smooth: gradient smoothing span (0: none) */
+    double vpmax;       /* optim.vpmax: the scaled gradient's maximum magnitude */
+} swit_precond;
+
+/* Bytes of device workspace swit_grad_precond needs for these parameters. */
+int swit_grad_precond_work(const swit_precond *p, size_t *bytes);
+
+/* grad_precond (postprocess.py:20-86) on the device, FP64 as the reference's numpy:
+ * grad, forw, back: float32 [nx][nz] (the solver's kernel and illuminations, as the
+ * reference reads them from its .bin files); mask: float64 [nx][nz] or NULL;
+ * out: float64 [nx][nz]; work: device memory of swit_grad_precond_work bytes.
+ * Smoothing (tools.py:255-275) runs as two exact-arithmetic-equivalent 1-D passes, so
+ * results equal the numpy restatement to FP64 rounding. */
+int swit_grad_precond(const swit_precond *p, const float *grad, const float *forw, const float *back,
+                      const double *mask, double *out, void *work, size_t work_bytes, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,8 @@ EXPORTS = (
     "swit_group_destroy",
     "swit_plan_set_option",
     "swit_plan_kernel_time",
+    "swit_grad_precond_work",
+    "swit_grad_precond",
 )
 
 SWIT_MISFIT_WAVEFORM = 0
@@ -55,6 +57,11 @@ SWIT_OPT_FWD_COLGROUP = 8
 SWIT_OPT_BWD_COLGROUP = 16
 SWIT_OPT_BWD_STEPS = 32
 SWIT_OPT_EXACT_FUSED = 64
+
+
+class SwitPrecond(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("nz", ctypes.c_int), ("marine", ctypes.c_int), ("grad_mute", ctypes.c_int),
+                ("grad_thred", ctypes.c_float), ("grad_smooth", ctypes.c_int), ("vpmax", ctypes.c_double)]
 
 
 class SwitGrid(ctypes.Structure):
@@ -124,6 +131,10 @@ def load(path: str | None = None):
     L.swit_plan_autotune.restype = ctypes.c_int
     L.swit_misfit.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, vp, vp, vp, vp, vp, vp]
     L.swit_misfit.restype = ctypes.c_int
+    L.swit_grad_precond_work.argtypes = [ctypes.POINTER(SwitPrecond), ctypes.POINTER(ctypes.c_size_t)]
+    L.swit_grad_precond_work.restype = ctypes.c_int
+    L.swit_grad_precond.argtypes = [ctypes.POINTER(SwitPrecond), vp, vp, vp, vp, vp, vp, ctypes.c_size_t, vp]
+    L.swit_grad_precond.restype = ctypes.c_int
     pp = ctypes.POINTER(vp)
     L.swit_group_create.argtypes = [pp, ctypes.c_int, pp]
     L.swit_group_create.restype = ctypes.c_int

@@ -19,6 +19,7 @@
 #include "fd_batched.hip"
 #include "fd_misfit.hip"
 #include "fd_exact.hip"
+#include "fd_precond.hip"
 
 using namespace swit;
 
@@ -1500,6 +1501,102 @@ extern "C" int swit_adjoint_exact(swit_plan *pl, const float *gseis, double *gra
     LAUNCH_CHECK();
     hipLaunchKernelGGL(ex::k_ex_fold, dim3(cdiv(d.nz, 64), cdiv(d.nx, 4)), dim3(64, 4), 0, stream, d, pl->c_int,
                        pl->rho_int, bs.gsum, grad);
+    LAUNCH_CHECK();
+    return SWIT_OK;
+}
+
+// ---- gradient preconditioning -----------------------------------------------------------
+
+static int precond_spans(const swit_precond *p, int *sill) {
+    const int mn = std::min(p->nx, p->nz);
+    *sill = mn > 40 ? 40 : mn / 2;                     // postprocess.py:62-65
+    return std::max(std::max(*sill, p->grad_smooth), p->grad_mute / 2);
+}
+
+extern "C" int swit_grad_precond_work(const swit_precond *p, size_t *bytes) {
+    g_err[0] = 0;
+    if (!p || !bytes) return set_err(SWIT_EINVAL, "parameters and bytes are required");
+    if (p->nx < 1 || p->nz < 1) return set_err(SWIT_EINVAL, "nx and nz must be positive");
+    int sill;
+    const int smax = precond_spans(p, &sill);
+    *bytes = sizeof(double) * ((size_t)4 * p->nx * p->nz + 3 * (size_t)(2 * smax + 1) + 16);
+    return SWIT_OK;
+}
+
+extern "C" int swit_grad_precond(const swit_precond *p, const float *grad, const float *forw, const float *back,
+                                 const double *mask, double *out, void *work, size_t work_bytes,
+                                 hipStream_t stream) {
+    g_err[0] = 0;
+    if (!p || !grad || !forw || !back || !out || !work)
+        return set_err(SWIT_EINVAL, "parameters, grad, forw, back, out and work are required");
+    if (p->nx < 1 || p->nz < 1) return set_err(SWIT_EINVAL, "nx and nz must be positive");
+    if (p->grad_mute < 0 || p->grad_mute > p->nz || p->grad_smooth < 0)
+        return set_err(SWIT_EINVAL, "grad_mute must be in [0, nz] and grad_smooth >= 0");
+    size_t need;
+    TRY(swit_grad_precond_work(p, &need));
+    if (work_bytes < need) return set_err(SWIT_EINVAL, "work is %zu bytes, needs %zu", work_bytes, need);
+    const int nx = p->nx, nz = p->nz;
+    const size_t n = (size_t)nx * nz;
+    int sill;
+    const int smax = precond_spans(p, &sill);
+    double *A = static_cast<double *>(work), *B = A + n, *C = B + n, *T = C + n;
+    double *f_ill = T + n, *f_grad = f_ill + (2 * smax + 1), *f_tap = f_grad + (2 * smax + 1);
+    double *M = f_tap + (2 * smax + 1);                // maxima: forw, back, precond, taper, |grad|
+    const dim3 b2(256), g2(cdiv(nz, 256), nx);
+    const int ge = (int)std::min<size_t>(cdiv((int)std::min<size_t>(n, (size_t)1 << 30), 256), 2048);
+    // smooth2d of the field in X (in place through T), rows z0.. nz-1
+    auto smooth = [&](double *X, int z0, int span, const double *f) -> int {
+        const dim3 g(cdiv(nz - z0, 256), nx);
+        hipLaunchKernelGGL(pc::k_smooth_z, g, b2, 0, stream, nx, nz, z0, span, f, X, T);
+        hipLaunchKernelGGL(pc::k_smooth_x, g, b2, 0, stream, nx, nz, z0, span, f, T, X);
+        LAUNCH_CHECK();
+        return SWIT_OK;
+    };
+    hipLaunchKernelGGL(pc::k_weights, dim3(1), dim3(1), 0, stream, sill, f_ill);
+    // illumination preconditioner (postprocess.py:62-75)
+    hipLaunchKernelGGL(pc::k_to_double, dim3(ge), dim3(256), 0, stream, n, forw, A);
+    hipLaunchKernelGGL(pc::k_to_double, dim3(ge), dim3(256), 0, stream, n, back, B);
+    LAUNCH_CHECK();
+    TRY(smooth(A, 0, sill, f_ill));
+    TRY(smooth(B, 0, sill, f_ill));
+    hipLaunchKernelGGL(pc::k_max, dim3(1), dim3(pc::kMaxThreads), 0, stream, n, A, 0, M);
+    hipLaunchKernelGGL(pc::k_max, dim3(1), dim3(pc::kMaxThreads), 0, stream, n, B, 0, M + 1);
+    hipLaunchKernelGGL(pc::k_precond_sum, dim3(ge), dim3(256), 0, stream, n, A, B, M, C);
+    hipLaunchKernelGGL(pc::k_max, dim3(1), dim3(pc::kMaxThreads), 0, stream, n, C, 0, M + 2);
+    LAUNCH_CHECK();
+    // taper (postprocess.py:56-58, 88-111): marine zeroes the rows above the mute, land
+    // damps them with a smoothed Hamming ramp
+    int kind = 0;
+    if (p->grad_mute > 0) {
+        if (p->marine) {
+            kind = 1;
+        } else {
+            kind = 2;
+            const int ts = p->grad_mute;
+            hipLaunchKernelGGL(pc::k_weights, dim3(1), dim3(1), 0, stream, ts / 2, f_tap);
+            hipLaunchKernelGGL(pc::k_taper0, g2, b2, 0, stream, nx, nz, ts, A);
+            LAUNCH_CHECK();
+            TRY(smooth(A, 0, ts / 2, f_tap));
+            hipLaunchKernelGGL(pc::k_max, dim3(1), dim3(pc::kMaxThreads), 0, stream, n, A, 0, M + 3);
+            hipLaunchKernelGGL(pc::k_taper_land, dim3(ge), dim3(256), 0, stream, n, (double)p->grad_thred, M + 3, A);
+            LAUNCH_CHECK();
+        }
+    }
+    hipLaunchKernelGGL(pc::k_apply, g2, b2, 0, stream, nx, nz, kind, p->grad_mute, grad, A, C, M, out);
+    LAUNCH_CHECK();
+    // gradient smoothing (postprocess.py:77-83): below the mute for marine data
+    if (p->grad_smooth > 0) {
+        const int z0 = p->marine ? p->grad_mute : 0;
+        hipLaunchKernelGGL(pc::k_weights, dim3(1), dim3(1), 0, stream, p->grad_smooth, f_grad);
+        LAUNCH_CHECK();
+        if (z0 < nz) TRY(smooth(out, z0, p->grad_smooth, f_grad));
+    }
+    if (mask) {
+        hipLaunchKernelGGL(pc::k_mul, dim3(ge), dim3(256), 0, stream, n, mask, out);
+        LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(pc::k_max, dim3(1), dim3(pc::kMaxThreads), 0, stream, n, out, 1, M + 4);
+    hipLaunchKernelGGL(pc::k_scale, dim3(ge), dim3(256), 0, stream, n, p->vpmax, M + 4, out);
     LAUNCH_CHECK();
     return SWIT_OK;
 }

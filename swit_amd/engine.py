@@ -289,3 +289,38 @@ def misfit(kind, obs, syn, dt):
                                   ctypes.c_void_p(s.data_ptr()), ctypes.c_void_p(adj.data_ptr()),
                                   ctypes.c_void_p(work.data_ptr()), ctypes.c_void_p(out.data_ptr()), stream))
     return out[0], adj, work.view(o.shape[:-1])
+
+
+def grad_precond(grad, forw, back, vpmax, marine, grad_mute=0, grad_thred=0.0, grad_smooth=0, mask=None):
+    """grad_precond (toolbox/postprocess.py:20-86) on the device: grad, forw, back (nx, nz)
+    float32 (as the solver's .bin files hold them; host or device), mask (nx, nz) or None.
+    Returns the preconditioned gradient as a float64 (nx, nz) device tensor."""
+    torch = _require_gpu()
+    lib = _native.load()
+    dev = torch.device("cuda", torch.cuda.current_device())
+
+    def f32(a):
+        return torch.as_tensor(np.asarray(a) if not torch.is_tensor(a) else a).to(dev, torch.float32).contiguous()
+    g, fw, bk = f32(grad), f32(forw), f32(back)
+    if g.dim() != 2 or fw.shape != g.shape or bk.shape != g.shape:
+        raise ValueError("grad, forw and back must be (nx, nz) arrays of one shape")
+    nx, nz = int(g.shape[0]), int(g.shape[1])
+    m = None
+    if mask is not None:
+        m = torch.as_tensor(np.asarray(mask) if not torch.is_tensor(mask) else mask).to(dev, torch.float64)
+        if tuple(m.shape) != (nx, nz):
+            raise ValueError('Wrong size of grad mask: the size of the mask should be identical to the size of vp model')
+        m = m.contiguous()
+    prm = _native.SwitPrecond(nx, nz, int(bool(marine)), int(grad_mute), float(grad_thred), int(grad_smooth),
+                              float(vpmax))
+    nbytes = ctypes.c_size_t()
+    _native.check(lib.swit_grad_precond_work(ctypes.byref(prm), ctypes.byref(nbytes)))
+    work = torch.empty(int(nbytes.value), dtype=torch.uint8, device=dev)
+    out = torch.empty((nx, nz), dtype=torch.float64, device=dev)
+    stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    _native.check(lib.swit_grad_precond(ctypes.byref(prm), ctypes.c_void_p(g.data_ptr()),
+                                        ctypes.c_void_p(fw.data_ptr()), ctypes.c_void_p(bk.data_ptr()),
+                                        ctypes.c_void_p(m.data_ptr()) if m is not None else None,
+                                        ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(work.data_ptr()),
+                                        nbytes.value, stream))
+    return out

@@ -19,7 +19,9 @@ def grad_precond(simu, optim, grad, forw, back):
     grad_thred = optim.grad_thred
     grad_smooth = optim.grad_smooth
     grad_mask = optim.grad_mask
-    grad = vector2array(np.asarray(grad), nx, nz).astype(np.float64)
+    # the gradient keeps its dtype (float32 from the solver's files) through the in-place
+    # taper product, as in the reference; the division by the preconditioner promotes it
+    grad = np.array(vector2array(np.asarray(grad), nx, nz))
     forw = vector2array(np.asarray(forw), nx, nz)
     back = vector2array(np.asarray(back), nx, nz)
     if grad_mute > 0:
@@ -45,6 +47,18 @@ def grad_precond(simu, optim, grad, forw, back):
         grad = grad * grad_mask
     grad *= vpmax / abs(grad).max()
     return array2vector(grad)
+
+
+def grad_precond_device(simu, optim, grad, forw, back):
+    """grad_precond on the GPU (swit_amd.engine.grad_precond, the same steps in FP64, the
+    smoothing as two 1-D passes): equal to grad_precond to FP64 rounding."""
+    from .engine import grad_precond as _device
+    nx, nz = simu.model.nx, simu.model.nz
+    g = _device(vector2array(np.asarray(grad, np.float32), nx, nz), vector2array(np.asarray(forw, np.float32), nx, nz),
+                vector2array(np.asarray(back, np.float32), nx, nz), optim.vpmax,
+                optim.marine_or_land in ['Marine', 'Offshore'], optim.grad_mute, optim.grad_thred,
+                optim.grad_smooth, optim.grad_mask)
+    return array2vector(g.cpu().numpy())
 
 
 def grad_taper(nx, nz, tapersize=20, thred=0.05, marine_or_land='Marine'):

@@ -37,6 +37,9 @@ WRITE_PARFILES = False
 PATH = "batched"
 # Boundary-strip storage: 'f32' (reference precision), 'f16' or 'bf16'.
 STRIPS = "f32"
+# grad_precond of adjoint(): 'device' (GPU kernels, equal to the numpy steps to FP64
+# rounding) or 'host' (the numpy restatement of toolbox/postprocess.py).
+PRECOND = "device"
 
 _JOBS = {}
 
@@ -196,6 +199,8 @@ def adjoint(simu, optim):
     grad = loadbinfloat32(homepath + 'data/syn/src0_kernel_vp.bin')
     forw = loadbinfloat32(homepath + 'data/syn/src0_illum_forw.bin')
     back = loadbinfloat32(homepath + 'data/syn/src0_illum_back.bin')
+    if PRECOND == 'device':
+        return _postprocess_module().grad_precond_device(simu, optim, grad, forw, back)
     return _postprocess_module().grad_precond(simu, optim, grad, forw, back)
 
 

@@ -65,7 +65,9 @@ def test_dropin_sequence_matches_oracle(tmp_path):
     ref = postprocess.grad_precond(simu, optim, a["grad"].astype(np.float32).ravel(),
                                    a["illum_forw"].astype(np.float32).ravel(),
                                    a["illum_back"].astype(np.float32).ravel())
-    np.testing.assert_array_equal(g, ref)
+    # adjoint() preconditions on the device (solver.PRECOND = 'device'): the numpy steps to
+    # FP64 rounding
+    assert np.linalg.norm(g - ref) <= 1e-12 * np.linalg.norm(ref)
     r = f["seis"].astype(np.float64) - obs
     assert J == pytest.approx(sum(misfit.misfit_waveform(obs[i], f["seis"][i], inp["dt"]) for i in range(w.ns)))
     assert J > 0

@@ -197,6 +197,7 @@ def test_dropin_plan_bookkeeping(tmp_path, monkeypatch):
     from swit_amd import engine, preprocess, solver, workloads
     from test_gpu_dropin import _optim, _simu
     monkeypatch.setattr(engine, "Plan", _RecordingPlan)
+    monkeypatch.setattr(solver, "PRECOND", "host")     # no GPU here for the preconditioner
     _RecordingPlan.log = []
     log = _RecordingPlan.log
     w = workloads.small(seed=22, nx=40, nz=30, npml=8, nt=60, ns=2)
