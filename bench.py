@@ -11,7 +11,9 @@ Prints one JSON line with the metric, a roofline object for the dominant kernel 
 backward step kernel, which does two reverse time steps per launch unless the autotuner
 picks the one-step kernel; its average duration comes from HIP events on the launch
 stream that bracket each time loop of the timed steps) and a CPU baseline (the FP32
-oracle on the host cores, bounded sample).
+oracle on the host cores, bounded sample).  Side legs, outside the timed steps: the
+plain per-shot path on a sample of shots, the same gradient through torch.autograd
+with the exact adjoint, and the four device misfits on the step's gathers.
 """
 from __future__ import annotations
 
@@ -99,6 +101,40 @@ def autograd_leg(w, src, obs, vp_init, rho_init, args):
             "algorithmic_bytes_per_gradient": int(per_grad), "achieved_gbs": round(gbs, 1),
             "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
             "finite": bool(torch.isfinite(g).all())}
+
+
+def misfit_leg(obs, syn, dt, reps=5):
+    """Device misfits and adjoint sources (swit_misfit) of every kind on the step's
+    gathers [ns][ng][nt]: average time per call from HIP events on torch's current
+    stream (the stream the misfit kernels are launched on), the HBM rate of the
+    algorithmic bytes (read obs and syn, write the adjoint source: 12 bytes a sample),
+    and the host restatement (numpy/scipy, as the reference computes them) timed on
+    one shot and scaled to all of them."""
+    import torch
+    from swit_amd import misfit as host
+    from swit_amd.engine import misfit
+    ns, nt = obs.shape[0], obs.shape[-1]
+    nbytes = 12 * obs.numel()
+    o1, s1 = obs[0].cpu().numpy(), syn[0].cpu().numpy()
+    out = {}
+    for kind in ("Waveform", "Globalcorrelation", "Envelope", "Traveltime"):
+        misfit(kind, obs, syn, dt)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            J, _, _ = misfit(kind, obs, syn, dt)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        t0 = time.perf_counter()
+        host._MISFIT[kind.lower()](o1, s1, dt)
+        host._ADJ[kind.lower()](o1, s1, dt)
+        host_ms = 1000 * (time.perf_counter() - t0) * ns
+        out[kind.lower()] = {"ms": round(ms, 3), "achieved_gbs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                             "host_ms_all_shots": round(host_ms, 1), "misfit": float(J)}
+    out["traces"] = int(obs.numel() // nt)
+    out["nt"] = int(nt)
+    return out
 
 
 def cpu_baseline(w, src, obs_syn_adj, nshots):
@@ -239,6 +275,8 @@ def main():
                     "time_loops_gbs": round((kern["forward"]["step_bytes"] * f_n + kern["backward"]["step_bytes"]
                                              * b_n) / ((f_ms + b_ms) * 1e-3) / 1e9, 1)}
 
+    misfits = misfit_leg(obs, plan.forward(src, save_strips=False), w.dt) if args.path == "batched" else None
+
     autograd = None
     if args.path == "batched" and not args.no_autograd:
         plan.close()
@@ -287,6 +325,7 @@ def main():
         "cpu_baseline": cpu,
         "plain_baseline": plain,
         "autograd": autograd,
+        "device_misfits": misfits,
     }
     if plain and autograd and "value" in autograd:
         autograd["over_plain"] = round(autograd["value"] / plain["value"], 2)

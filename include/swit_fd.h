@@ -174,10 +174,18 @@ int swit_adjoint_exact(swit_plan *plan, const float *gseis, double *grad, hipStr
  *   SWIT_MISFIT_WAVEFORM: adj = syn - obs, misfit = sum sqrt(sum r^2 dt)
  *     (misfit.py:128-148, :263-282);
  *   SWIT_MISFIT_GLOBAL_CORRELATION: o = obs/|obs|, s = syn/|syn|, c = Pearson(s, o),
- *     misfit = -sum c, adj = (s c - o) / |syn| (misfit.py:219-242, :338-355).
+ *     misfit = -sum c, adj = (s c - o) / |syn| (misfit.py:219-242, :338-355);
+ *   SWIT_MISFIT_ENVELOPE: power-2 envelope difference with the Hilbert transform of
+ *     scipy.signal.hilbert, misfit = dt sum r^2 (misfit.py:151-185, :286-308);
+ *   SWIT_MISFIT_TRAVELTIME: cross-correlation lag from the FFT correlation maximum,
+ *     misfit = 0.5 lag^2 dt, adj = scaled central difference of syn
+ *     (misfit.py:188-215, :312-334, :359-373); traces whose difference is all zero
+ *     are dead too.  These two take nt <= 9216 and run their transforms in FP64.
  * Sums run in FP64; the sum over traces has a fixed order (reproducible). */
 #define SWIT_MISFIT_WAVEFORM 0
 #define SWIT_MISFIT_GLOBAL_CORRELATION 1
+#define SWIT_MISFIT_ENVELOPE 2
+#define SWIT_MISFIT_TRAVELTIME 3
 int swit_misfit(int kind, int ntraces, int nt, float dt, const float *obs, const float *syn, float *adj,
                 double *work, double *misfit, hipStream_t stream);
 

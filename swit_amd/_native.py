@@ -50,6 +50,8 @@ EXPORTS = (
 
 SWIT_MISFIT_WAVEFORM = 0
 SWIT_MISFIT_GLOBAL_CORRELATION = 1
+SWIT_MISFIT_ENVELOPE = 2
+SWIT_MISFIT_TRAVELTIME = 3
 SWIT_OPT_GRAPH = 1
 SWIT_OPT_PROFILE = 2
 SWIT_OPT_FWD_GROUP = 4

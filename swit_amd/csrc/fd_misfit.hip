@@ -7,13 +7,25 @@
 //     arrays), misfit = sum_traces sqrt(sum r^2 dt);
 //   global correlation (:219-242, :338-355): o = obs/|obs|, s = syn/|syn|,
 //     c = Pearson coefficient of (s, o) (np.corrcoef, means removed),
-//     misfit = -sum_traces c, adj = (s c - o) / |syn|.
+//     misfit = -sum_traces c, adj = (s c - o) / |syn|;
+//   envelope, power 2 (:151-185, :286-308): E = x^2 + H(x)^2 with H the discrete
+//     Hilbert transform of scipy.signal.hilbert, r = E_syn - E_obs,
+//     misfit = dt sum_traces sum r^2, adj = 2 s r - H(2 H(s) r);
+//   traveltime (:188-215, :312-334, :359-373): lag = argmax |ifft(fft(obs) conj(-fft(syn)))|
+//     folded as cross_correlate_max, d = central difference of syn (FP32 as the
+//     reference's float32 traces), misfit = 0.5 (lag - nt + 1)^2 dt,
+//     adj = d / (sum d^2 dt) * (lag - nt + 1) dt; a trace with sum |d| = 0 is dead.
+// The reference takes its FFTs of float32 traces in single precision (scipy keeps
+// float32 input in complex64); here every transform runs in FP64.
 #include "fd_common.h"
 
 namespace swit {
 
 constexpr int kMisfitThreads = 256;
 
+// Sum over a workgroup of T threads, the same fixed order on every call; every
+// thread gets the sum.  sh holds T/64 doubles.
+template <int T = kMisfitThreads>
 __device__ __forceinline__ double block_sum(double v, double *sh) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -23,7 +35,7 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
     __syncthreads();
     double t = 0.0;
 #pragma unroll
-    for (int w = 0; w < kMisfitThreads / 64; ++w) t += sh[w];
+    for (int w = 0; w < T / 64; ++w) t += sh[w];
     return t;
 }
 
@@ -91,6 +103,271 @@ __global__ __launch_bounds__(kMisfitThreads) void k_sum_fixed(int n, const doubl
     for (int i = threadIdx.x; i < n; i += kMisfitThreads) t += v[i];
     t = block_sum(t, sh);
     if (threadIdx.x == 0) out[0] = t;
+}
+
+// ---- FFT misfits ------------------------------------------------------------------------
+// One workgroup per trace holds the trace as nt complex doubles in LDS and transforms
+// it in place.  The FFT is a Stockham autosort over the radices of nt (any factor; 2s
+// are paired to 4), written as a gather: each thread computes its outputs of a stage
+// into registers from the stage's inputs, then all threads store them.  Output o of a
+// stage with radix R after ns points of earlier stages is
+//   sum_r x[j + r nt/R] W^(r (jm + q ns) nt/(ns R)),  jm = o mod ns, q = (o/ns) mod R,
+//   j = (o / (ns R)) ns + jm,  W = exp(-+2 pi i / nt),
+// and twiddles come from two LDS tables of about sqrt(nt) entries each:
+// W^e = A[e / s] B[e mod s].  Two real traces go through one complex transform.
+constexpr int kFftMaxStages = 24;
+constexpr int kFftMaxN = 9216;
+constexpr int kFftTw = 96;      // ceil(sqrt(kFftMaxN)) and ceil(kFftMaxN / that)
+
+struct FftFactors {
+    int n, nst, s;              // length, stages, twiddle split (s = ceil(sqrt(n)))
+    int radix[kFftMaxStages];
+};
+
+// Threads per workgroup for a transform of at most NMAX points: the largest size runs
+// at 512 so that its per-thread outputs fit in registers.
+template <int NMAX>
+constexpr int fft_threads() { return NMAX > 4096 ? 512 : 1024; }
+
+template <int NMAX>
+struct FftLds {
+    static constexpr int T = fft_threads<NMAX>();
+    double2 x[NMAX];
+    double2 ta[kFftTw], tb[kFftTw];      // exp(-2 pi i a s / n), exp(-2 pi i b / n)
+    double sh[T / 64];
+    int ish[T / 64];
+};
+
+template <int NMAX>
+__device__ void fft_twiddles(FftLds<NMAX> &L, const FftFactors &f) {
+    constexpr int T = fft_threads<NMAX>();
+    const int na = (f.n + f.s - 1) / f.s;
+    for (int i = threadIdx.x; i < na + f.s; i += T) {
+        const int e = i < na ? i * f.s : i - na;
+        double sv, cv;
+        sincospi(2.0 * (double)e / (double)f.n, &sv, &cv);
+        if (i < na) L.ta[i] = make_double2(cv, -sv);
+        else L.tb[e] = make_double2(cv, -sv);
+    }
+}
+
+// In-place DFT of L.x[0..n) (INV: inverse without the 1/n factor).  Begins and ends
+// with a barrier.
+template <int NMAX, bool INV>
+__device__ void fft_lds(FftLds<NMAX> &L, const FftFactors &f) {
+    constexpr int T = fft_threads<NMAX>(), E = (NMAX + T - 1) / T;
+    const int n = f.n, s = f.s, nh = n / s, nl = n % s;
+    int ns = 1;
+    __syncthreads();
+    for (int st = 0; st < f.nst; ++st) {
+        const int R = f.radix[st], nr = n / R, span = ns * R, mult = n / span;
+        double2 acc[E];
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const int o = threadIdx.x + k * T;
+            double re = 0.0, im = 0.0;
+            if (o < n) {
+                const int jm = o % ns, q = (o / ns) % R;
+                const int j = (o / span) * ns + jm;
+                const int step = (jm + q * ns) * mult;        // < n
+                const int sh = step / s, sl = step % s;
+                int eh = 0, el = 0;                           // exponent eh s + el, kept < n
+                for (int r = 0; r < R; ++r) {
+                    const double2 v = L.x[j + r * nr];
+                    const double2 a = L.ta[eh], b = L.tb[el];
+                    double wr = a.x * b.x - a.y * b.y, wi = a.x * b.y + a.y * b.x;
+                    if (INV) wi = -wi;
+                    re += v.x * wr - v.y * wi;
+                    im += v.x * wi + v.y * wr;
+                    el += sl;
+                    eh += sh;
+                    if (el >= s) { el -= s; ++eh; }
+                    if (eh > nh || (eh == nh && el >= nl)) {
+                        eh -= nh;
+                        el -= nl;
+                        if (el < 0) { el += s; --eh; }
+                    }
+                }
+            }
+            acc[k] = make_double2(re, im);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const int o = threadIdx.x + k * T;
+            if (o < n) L.x[o] = acc[k];
+        }
+        __syncthreads();
+        ns = span;
+    }
+}
+
+// x <- -i sgn(k) x, sgn as scipy.signal.hilbert's h - 1: 0 at k = 0 (and k = n/2 for
+// even n), +1 below n/2, -1 above.  After the inverse transform, the real and
+// imaginary parts are the Hilbert transforms of the two packed real traces.
+template <int NMAX>
+__device__ void hilbert_multiplier(FftLds<NMAX> &L, int n) {
+    constexpr int T = fft_threads<NMAX>();
+    for (int k = threadIdx.x; k < n; k += T) {
+        const double sg = (k == 0 || 2 * k == n) ? 0.0 : (2 * k < n ? 1.0 : -1.0);
+        const double2 v = L.x[k];
+        L.x[k] = make_double2(sg * v.y, -sg * v.x);
+    }
+}
+
+// Envelope misfit (power 2) of one trace per workgroup.
+template <int NMAX>
+__global__ __launch_bounds__(fft_threads<NMAX>()) void k_misfit_envelope(FftFactors f, float dt,
+                                                                 const float *__restrict__ obs,
+                                                                 const float *__restrict__ syn,
+                                                                 float *__restrict__ adj,
+                                                                 double *__restrict__ tmis) {
+    constexpr int T = fft_threads<NMAX>(), E = (NMAX + T - 1) / T;
+    __shared__ FftLds<NMAX> L;
+    const int n = f.n;
+    const size_t base = (size_t)blockIdx.x * n;
+    const float *o = obs + base, *s = syn + base;
+    float *a = adj + base;
+    fft_twiddles(L, f);
+    double so = 0.0, ss = 0.0;
+    for (int i = threadIdx.x; i < n; i += T) {
+        const double ov = o[i], sv = s[i];
+        so += ov * ov;
+        ss += sv * sv;
+        L.x[i] = make_double2(sv, ov);
+    }
+    so = block_sum<T>(so, L.sh);
+    ss = block_sum<T>(ss, L.sh);
+    if (!(so > 0.0 && ss > 0.0)) {                    // uniform over the workgroup
+        for (int i = threadIdx.x; i < n; i += T) a[i] = 0.0f;
+        if (threadIdx.x == 0) tmis[blockIdx.x] = 0.0;
+        return;
+    }
+    fft_lds<NMAX, false>(L, f);
+    hilbert_multiplier(L, n);
+    fft_lds<NMAX, true>(L, f);
+    const double inv_n = 1.0 / n;
+    double part[E];
+    double r2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int i = threadIdx.x + k * T;
+        part[k] = 0.0;
+        if (i < n) {
+            const double2 h = L.x[i];
+            const double sh = h.x * inv_n, oh = h.y * inv_n;
+            const double sv = s[i], ov = o[i];
+            const double r = (sv * sv + sh * sh) - (ov * ov + oh * oh);
+            r2 += r * r;
+            part[k] = 2.0 * sv * r;
+            L.x[i] = make_double2(2.0 * sh * r, 0.0);
+        }
+    }
+    r2 = block_sum<T>(r2, L.sh);
+    fft_lds<NMAX, false>(L, f);
+    hilbert_multiplier(L, n);
+    fft_lds<NMAX, true>(L, f);
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int i = threadIdx.x + k * T;
+        if (i < n) a[i] = (float)(part[k] - L.x[i].x * inv_n);
+    }
+    if (threadIdx.x == 0) tmis[blockIdx.x] = r2 * (double)dt;
+}
+
+// Cross-correlation traveltime misfit of one trace per workgroup.
+template <int NMAX>
+__global__ __launch_bounds__(fft_threads<NMAX>()) void k_misfit_traveltime(FftFactors f, float dt,
+                                                                   const float *__restrict__ obs,
+                                                                   const float *__restrict__ syn,
+                                                                   float *__restrict__ adj,
+                                                                   double *__restrict__ tmis) {
+    constexpr int T = fft_threads<NMAX>(), E = (NMAX + T - 1) / T;
+    __shared__ FftLds<NMAX> L;
+    const int n = f.n;
+    const size_t base = (size_t)blockIdx.x * n;
+    const float *o = obs + base, *s = syn + base;
+    float *a = adj + base;
+    const float two_dt = 2.0f * dt;
+    fft_twiddles(L, f);
+    double so = 0.0, ss = 0.0, dabs = 0.0, d2 = 0.0;
+    for (int i = threadIdx.x; i < n; i += T) {
+        const double ov = o[i], sv = s[i];
+        so += ov * ov;
+        ss += sv * sv;
+        L.x[i] = make_double2(ov, sv);
+        if (i >= 1 && i + 1 < n) {
+            const double d = (s[i + 1] - s[i - 1]) / two_dt;
+            dabs += fabs(d);
+            d2 += d * d;
+        }
+    }
+    so = block_sum<T>(so, L.sh);
+    ss = block_sum<T>(ss, L.sh);
+    dabs = block_sum<T>(dabs, L.sh);
+    d2 = block_sum<T>(d2, L.sh);
+    if (!(so > 0.0 && ss > 0.0 && dabs > 0.0)) {     // uniform over the workgroup
+        for (int i = threadIdx.x; i < n; i += T) a[i] = 0.0f;
+        if (threadIdx.x == 0) tmis[blockIdx.x] = 0.0;
+        return;
+    }
+    fft_lds<NMAX, false>(L, f);
+    // Z = O + i S  ->  P = O * conj(-S), with O = (Z_k + conj Z_-k)/2, S = (Z_k - conj Z_-k)/(2i);
+    // P_-k = conj P_k.
+    double2 pk[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int i = threadIdx.x + k * T;
+        pk[k] = make_double2(0.0, 0.0);
+        if (2 * i <= n) {
+            const double2 z1 = L.x[i], z2 = L.x[i == 0 ? 0 : n - i];
+            const double orr = 0.5 * (z1.x + z2.x), oi = 0.5 * (z1.y - z2.y);
+            const double sr = 0.5 * (z1.y + z2.y), si = -0.5 * (z1.x - z2.x);
+            // O * conj(-S) = -(O * conj S)
+            pk[k] = make_double2(-(orr * sr + oi * si), -(oi * sr - orr * si));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int i = threadIdx.x + k * T;
+        if (2 * i <= n) {
+            L.x[i] = pk[k];
+            if (i != 0) L.x[n - i] = make_double2(pk[k].x, -pk[k].y);
+        }
+    }
+    fft_lds<NMAX, true>(L, f);
+    // first index of the largest |cc| (numpy argmax)
+    double best = -1.0;
+    int bi = 0;
+    for (int i = threadIdx.x; i < n; i += T) {
+        const double2 c = L.x[i];
+        const double m = c.x * c.x + c.y * c.y;
+        if (m > best) { best = m; bi = i; }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bi, off);
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) { L.sh[wave] = best; L.ish[wave] = bi; }
+    __syncthreads();
+    best = L.sh[0];
+    bi = L.ish[0];
+    for (int w = 1; w < T / 64; ++w)
+        if (L.sh[w] > best || (L.sh[w] == best && L.ish[w] < bi)) { best = L.sh[w]; bi = L.ish[w]; }
+    int cc = bi - 1;
+    if (cc < n / 2) cc += n;
+    const double lag = (double)(cc - n + 1);
+    const double inv = 1.0 / (d2 * (double)dt), sc = lag * (double)dt;
+    for (int i = threadIdx.x; i < n; i += T) {
+        double d = 0.0;
+        if (i >= 1 && i + 1 < n) d = (double)((s[i + 1] - s[i - 1]) / two_dt);
+        a[i] = (float)((d * inv) * sc);
+    }
+    if (threadIdx.x == 0) tmis[blockIdx.x] = 0.5 * lag * lag * (double)dt;
 }
 
 }  // namespace swit

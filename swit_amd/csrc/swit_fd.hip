@@ -1332,19 +1332,53 @@ extern "C" int swit_group_adjoint(swit_group *gr, const float *const *src, const
 
 // ---- misfits -------------------------------------------------------------------------------
 
+// Radices of an n-point transform: 4s first, then 2, 3 and the odd primes in order.
+static FftFactors fft_factors(int n) {
+    FftFactors f{};
+    f.n = n;
+    f.s = 1;
+    while (f.s * f.s < n) ++f.s;
+    int m = n;
+    for (int p : {4, 2, 3})
+        while (m % p == 0) { f.radix[f.nst++] = p; m /= p; }
+    for (int p = 5; m > 1; p += 2)
+        while (m % p == 0) { f.radix[f.nst++] = p; m /= p; }
+    return f;
+}
+
+template <int NMAX>
+static void launch_fft_misfit(int kind, const FftFactors &f, int ntraces, float dt, const float *obs,
+                              const float *syn, float *adj, double *work, hipStream_t stream) {
+    if (kind == SWIT_MISFIT_ENVELOPE)
+        hipLaunchKernelGGL(k_misfit_envelope<NMAX>, dim3(ntraces), dim3(fft_threads<NMAX>()), 0, stream, f, dt, obs, syn,
+                           adj, work);
+    else
+        hipLaunchKernelGGL(k_misfit_traveltime<NMAX>, dim3(ntraces), dim3(fft_threads<NMAX>()), 0, stream, f, dt, obs, syn,
+                           adj, work);
+}
+
 extern "C" int swit_misfit(int kind, int ntraces, int nt, float dt, const float *obs, const float *syn,
                            float *adj, double *work, double *misfit, hipStream_t stream) {
     g_err[0] = 0;
-    if (kind != SWIT_MISFIT_WAVEFORM && kind != SWIT_MISFIT_GLOBAL_CORRELATION)
+    if (kind != SWIT_MISFIT_WAVEFORM && kind != SWIT_MISFIT_GLOBAL_CORRELATION && kind != SWIT_MISFIT_ENVELOPE &&
+        kind != SWIT_MISFIT_TRAVELTIME)
         return set_err(SWIT_EINVAL, "unknown misfit kind %d", kind);
     if (ntraces < 1 || nt < 1) return set_err(SWIT_EINVAL, "need ntraces >= 1 and nt >= 1");
     if (!obs || !syn || !adj || !work || !misfit) return set_err(SWIT_EINVAL, "all arrays are required");
-    if (kind == SWIT_MISFIT_WAVEFORM)
+    if ((kind == SWIT_MISFIT_ENVELOPE || kind == SWIT_MISFIT_TRAVELTIME) && nt > kFftMaxN)
+        return set_err(SWIT_EINVAL, "envelope and traveltime misfits take nt <= %d (got %d)", kFftMaxN, nt);
+    if (kind == SWIT_MISFIT_WAVEFORM) {
         hipLaunchKernelGGL(k_misfit_trace<0>, dim3(ntraces), dim3(kMisfitThreads), 0, stream, nt, dt, obs, syn, adj,
                            work);
-    else
+    } else if (kind == SWIT_MISFIT_GLOBAL_CORRELATION) {
         hipLaunchKernelGGL(k_misfit_trace<1>, dim3(ntraces), dim3(kMisfitThreads), 0, stream, nt, dt, obs, syn, adj,
                            work);
+    } else {
+        const FftFactors f = fft_factors(nt);
+        if (nt <= 2048) launch_fft_misfit<2048>(kind, f, ntraces, dt, obs, syn, adj, work, stream);
+        else if (nt <= 4096) launch_fft_misfit<4096>(kind, f, ntraces, dt, obs, syn, adj, work, stream);
+        else launch_fft_misfit<kFftMaxN>(kind, f, ntraces, dt, obs, syn, adj, work, stream);
+    }
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(kMisfitThreads), 0, stream, ntraces, work, misfit);
     LAUNCH_CHECK();

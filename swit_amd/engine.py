@@ -263,15 +263,17 @@ class PlanGroup:
 
 
 _MISFIT_KIND = {"waveform": _native.SWIT_MISFIT_WAVEFORM,
-                "globalcorrelation": _native.SWIT_MISFIT_GLOBAL_CORRELATION}
+                "globalcorrelation": _native.SWIT_MISFIT_GLOBAL_CORRELATION,
+                "envelope": _native.SWIT_MISFIT_ENVELOPE,
+                "traveltime": _native.SWIT_MISFIT_TRAVELTIME}
 
 
 def misfit(kind, obs, syn, dt):
     """Misfit and adjoint source of device gathers (toolbox/misfit.py rules).
 
-    kind: 'Waveform' or 'Globalcorrelation'; obs, syn: float32 device tensors of one
-    shape (..., nt).  Returns (misfit as a float64 device scalar, adjoint source of
-    syn's shape, per-trace misfits)."""
+    kind: 'Waveform', 'Globalcorrelation', 'Envelope' or 'Traveltime'; obs, syn:
+    float32 device tensors of one shape (..., nt).  Returns (misfit as a float64
+    device scalar, adjoint source of syn's shape, per-trace misfits)."""
     torch = _require_gpu()
     lib = _native.load()
     code = _MISFIT_KIND[kind.lower()]

@@ -165,6 +165,32 @@ def test_envelope_zero_for_identical_traces():
     np.testing.assert_allclose(misfit.adjoint_source_envelope(obs, obs, 0.002), 0.0, atol=1e-12)
 
 
+def test_envelope_adjoint_is_the_misfit_gradient():
+    """The envelope misfit J = dt sum r^2 (r = E_syn - E_obs) has dJ/dsyn = 2 dt adj:
+    adj = 2 s r - H(2 H(s) r) uses that the discrete Hilbert transform is antisymmetric
+    (H^T = -H), so a dropped sign or an untransposed H fails this directional check."""
+    rng = np.random.default_rng(3)
+    obs = rng.standard_normal((3, 96))
+    syn = rng.standard_normal((3, 96))
+    dt = 0.004
+    adj = misfit.adjoint_source_envelope(obs, syn, dt)
+    ds = rng.standard_normal(syn.shape)
+    eps = 1e-6
+    fd = (misfit.misfit_envelope(obs, syn + eps * ds, dt) - misfit.misfit_envelope(obs, syn - eps * ds, dt)) / (2 * eps)
+    assert 2 * dt * np.sum(adj * ds) == pytest.approx(fd, rel=1e-7)
+
+
+def test_hilbert_transform_of_cosine_is_sine():
+    """scipy.signal.hilbert on a whole number of periods: H cos = sin, so the envelope is
+    flat (closed form used by the device envelope test)."""
+    from scipy.signal import hilbert
+    n = 240
+    t = np.arange(n)
+    for k in (1, 7, 119):
+        np.testing.assert_allclose(hilbert(np.cos(2 * np.pi * k * t / n)).imag, np.sin(2 * np.pi * k * t / n),
+                                   atol=1e-12)
+
+
 class _RecordingPlan:
     """Stands in for swit_amd.engine.Plan to check the drop-in's plan bookkeeping on the
     CPU: every call is logged, seismograms are zeros of the right shape."""
