@@ -13,7 +13,8 @@ picks the one-step kernel; its average duration comes from HIP events on the lau
 stream that bracket each time loop of the timed steps) and a CPU baseline (the FP32
 oracle on the host cores, bounded sample).  Side legs, outside the timed steps: the
 plain per-shot path on a sample of shots, the same gradient through torch.autograd
-with the exact adjoint, and the four device misfits on the step's gathers.
+with the exact adjoint, and the four device misfits and the device shot-gather
+processing on the step's gathers.
 """
 from __future__ import annotations
 
@@ -135,6 +136,39 @@ def misfit_leg(obs, syn, dt, reps=5):
     out["traces"] = int(obs.numel() // nt)
     out["nt"] = int(nt)
     return out
+
+
+def process_leg(syn, dt, offsets, reps=3):
+    """Device shot-gather processing (swit_process) of the step's gathers with the
+    reference's usual settings (first-break late mute, 2-12 Hz Butterworth bandpass
+    with detrend and taper, L2 trace normalisation): average time per call from HIP
+    events on torch's current stream, against the host restatement (numpy/scipy, the
+    reference's float32 flow) timed on one shot and scaled to all of them."""
+    import torch
+    from swit_amd import base, preprocess
+    from swit_amd.engine import process_gathers
+    o = base.optimize('Waveform', 'LBFGS', 1, 0.05, 6000.0, 1000.0, 'Marine', 0, 0, 'Bandpass', 2.0, 12.0, 1, 0.5,
+                      ['L2-Trace'], 0, 0, 0.0, 0.0)
+    x = syn.clone()
+    process_gathers(o, x, dt, offsets)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ms = 0.0
+    for _ in range(reps):
+        x.copy_(syn)
+        e0.record()
+        process_gathers(o, x, dt, offsets)
+        e1.record()
+        torch.cuda.synchronize()
+        ms += e0.elapsed_time(e1) / reps
+    s1 = syn[0].cpu().numpy()
+    t0 = time.perf_counter()
+    h = preprocess.apply_mute(o, s1, dt, np.asarray(offsets)[0])
+    h = preprocess.apply_filter(o, h, dt)
+    preprocess.apply_normalize(o, h)
+    host_ms = 1000 * (time.perf_counter() - t0) * syn.shape[0]
+    return {"ms": round(ms, 3), "host_ms_all_shots": round(host_ms, 1),
+            "steps": "late first-break mute, bandpass 2-12 Hz filtfilt + detrend + taper, L2-Trace",
+            "traces": int(syn.numel() // syn.shape[-1]), "nt": int(syn.shape[-1])}
 
 
 def cpu_baseline(w, src, obs_syn_adj, nshots):
@@ -275,7 +309,12 @@ def main():
                     "time_loops_gbs": round((kern["forward"]["step_bytes"] * f_n + kern["backward"]["step_bytes"]
                                              * b_n) / ((f_ms + b_ms) * 1e-3) / 1e9, 1)}
 
-    misfits = misfit_leg(obs, plan.forward(src, save_strips=False), w.dt) if args.path == "batched" else None
+    misfits = processing = None
+    if args.path == "batched":
+        syn_now = plan.forward(src, save_strips=False)
+        misfits = misfit_leg(obs, syn_now, w.dt)
+        offsets = np.round(w.rec_xz[:, :, 0]) - np.round(w.src_xz[:, None, 0])
+        processing = process_leg(syn_now, w.dt, offsets)
 
     autograd = None
     if args.path == "batched" and not args.no_autograd:
@@ -326,6 +365,7 @@ def main():
         "plain_baseline": plain,
         "autograd": autograd,
         "device_misfits": misfits,
+        "device_processing": processing,
     }
     if plain and autograd and "value" in autograd:
         autograd["over_plain"] = round(autograd["value"] / plain["value"], 2)

@@ -235,6 +235,50 @@ int swit_grad_precond_work(const swit_precond *p, size_t *bytes);
 int swit_grad_precond(const swit_precond *p, const float *grad, const float *forw, const float *back,
                       const double *mask, double *out, void *work, size_t work_bytes, hipStream_t stream);
 
+/* ---- shot-gather processing (toolbox/preprocess.py:23-200) ----------------------------
+ * process_workflow_serial's steps on device gathers, in its order: offset mutes and the
+ * first-break window mute (apply_mute, :98-151), Butterworth filtfilt followed by
+ * demean, linear detrend and a 5 % Hann taper (apply_filter, :64-95), trace and event
+ * normalisation (apply_normalize, :154-200).  Data stay float32 between steps as the
+ * reference's obspy traces do; the filter runs in FP64 with scipy's recurrence order. */
+#define SWIT_PROCESS_MAX_TAPS 16
+#define SWIT_NORM_NONE 0
+#define SWIT_NORM_L1 1
+#define SWIT_NORM_L2 2
+#define SWIT_NORM_MAX 3          /* trace normalisation only */
+
+typedef struct swit_process_params {
+    int ngathers, ntr, nt;       /* data [ngathers][ntr][nt]; an event is one gather */
+    double dt;                   /* sample interval, s */
+    int mute_offset_short;       /* zero traces with |offset| < mute_offset_short_dis */
+    int mute_offset_long;        /* zero traces with |offset| > mute_offset_long_dis */
+    double mute_offset_short_dis, mute_offset_long_dis;   /* m */
+    int mute_late_arrival;       /* window mute around the first break */
+    double mute_late_window;     /* s; > 0 mutes arrivals later than pick + window,
+                                    otherwise earlier ones (preprocess.py:121-125) */
+    int filter_taps;             /* len(b) = len(a), 2..16; 0: no filter */
+    double b[SWIT_PROCESS_MAX_TAPS], a[SWIT_PROCESS_MAX_TAPS];  /* a[0] = 1 */
+    double zi[SWIT_PROCESS_MAX_TAPS];   /* scipy.signal.lfilter_zi(b, a) */
+    int detrend_taper;           /* demean, linear detrend and taper (after the filter) */
+    int norm_trace;              /* SWIT_NORM_* */
+    int norm_event;              /* SWIT_NORM_NONE, _L1 or _L2 */
+} swit_process_params;
+
+/* Bytes of device workspace swit_process needs for these parameters. */
+int swit_process_work(const swit_process_params *p, size_t *bytes);
+
+/* Processes data [ngathers][ntr][nt] (device, float32) in place.
+ *   offsets: device float [ngathers*ntr], group x - source x per trace (the SU
+ *            header's offset; required with an offset mute);
+ *   mute_win: device double [100], sin(linspace(0, pi, 200))[:100] as numpy computes
+ *            it (required with the window mute);
+ *   taper:   device double [nt], obspy's Trace.taper(0.05, 'hann') weights
+ *            (required with detrend_taper);
+ *   work:    device memory of swit_process_work bytes.
+ * The filter needs nt > 3 * filter_taps (scipy's padding). */
+int swit_process(const swit_process_params *p, const float *offsets, const double *mute_win, const double *taper,
+                 float *data, void *work, size_t work_bytes, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,8 @@ EXPORTS = (
     "swit_plan_kernel_time",
     "swit_grad_precond_work",
     "swit_grad_precond",
+    "swit_process_work",
+    "swit_process",
 )
 
 SWIT_MISFIT_WAVEFORM = 0
@@ -64,6 +66,20 @@ SWIT_OPT_EXACT_FUSED = 64
 class SwitPrecond(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("nz", ctypes.c_int), ("marine", ctypes.c_int), ("grad_mute", ctypes.c_int),
                 ("grad_thred", ctypes.c_float), ("grad_smooth", ctypes.c_int), ("vpmax", ctypes.c_double)]
+
+
+SWIT_PROCESS_MAX_TAPS = 16
+SWIT_NORM = {"none": 0, "l1": 1, "l2": 2, "max": 3}
+
+
+class SwitProcessParams(ctypes.Structure):
+    _fields_ = [("ngathers", ctypes.c_int), ("ntr", ctypes.c_int), ("nt", ctypes.c_int), ("dt", ctypes.c_double),
+                ("mute_offset_short", ctypes.c_int), ("mute_offset_long", ctypes.c_int),
+                ("mute_offset_short_dis", ctypes.c_double), ("mute_offset_long_dis", ctypes.c_double),
+                ("mute_late_arrival", ctypes.c_int), ("mute_late_window", ctypes.c_double),
+                ("filter_taps", ctypes.c_int), ("b", ctypes.c_double * SWIT_PROCESS_MAX_TAPS),
+                ("a", ctypes.c_double * SWIT_PROCESS_MAX_TAPS), ("zi", ctypes.c_double * SWIT_PROCESS_MAX_TAPS),
+                ("detrend_taper", ctypes.c_int), ("norm_trace", ctypes.c_int), ("norm_event", ctypes.c_int)]
 
 
 class SwitGrid(ctypes.Structure):
@@ -137,6 +153,10 @@ def load(path: str | None = None):
     L.swit_grad_precond_work.restype = ctypes.c_int
     L.swit_grad_precond.argtypes = [ctypes.POINTER(SwitPrecond), vp, vp, vp, vp, vp, vp, ctypes.c_size_t, vp]
     L.swit_grad_precond.restype = ctypes.c_int
+    L.swit_process_work.argtypes = [ctypes.POINTER(SwitProcessParams), ctypes.POINTER(ctypes.c_size_t)]
+    L.swit_process_work.restype = ctypes.c_int
+    L.swit_process.argtypes = [ctypes.POINTER(SwitProcessParams), vp, vp, vp, vp, vp, ctypes.c_size_t, vp]
+    L.swit_process.restype = ctypes.c_int
     pp = ctypes.POINTER(vp)
     L.swit_group_create.argtypes = [pp, ctypes.c_int, pp]
     L.swit_group_create.restype = ctypes.c_int

@@ -20,6 +20,7 @@
 #include "fd_misfit.hip"
 #include "fd_exact.hip"
 #include "fd_precond.hip"
+#include "fd_process.hip"
 
 using namespace swit;
 
@@ -1632,5 +1633,114 @@ extern "C" int swit_grad_precond(const swit_precond *p, const float *grad, const
     hipLaunchKernelGGL(pc::k_max, dim3(1), dim3(pc::kMaxThreads), 0, stream, n, out, 1, M + 4);
     hipLaunchKernelGGL(pc::k_scale, dim3(ge), dim3(256), 0, stream, n, p->vpmax, M + 4, out);
     LAUNCH_CHECK();
+    return SWIT_OK;
+}
+
+// ---- shot-gather processing --------------------------------------------------------------
+
+namespace {
+constexpr int kProcChunk = 16384;   // traces per filtfilt launch (workspace 8 (nt + 6 taps) each)
+
+int process_check(const swit_process_params *p) {
+    if (!p) return set_err(SWIT_EINVAL, "parameters are required");
+    if (p->ngathers < 1 || p->ntr < 1 || p->nt < 1) return set_err(SWIT_EINVAL, "need ngathers, ntr, nt >= 1");
+    if (p->filter_taps) {
+        if (p->filter_taps < 2 || p->filter_taps > SWIT_PROCESS_MAX_TAPS)
+            return set_err(SWIT_EINVAL, "filter_taps must be 2..%d", SWIT_PROCESS_MAX_TAPS);
+        if (p->a[0] != 1.0) return set_err(SWIT_EINVAL, "filter coefficients must have a[0] = 1");
+        if (p->nt <= 3 * p->filter_taps)
+            return set_err(SWIT_EINVAL, "the filter needs nt > %d (3 * taps)", 3 * p->filter_taps);
+    }
+    if (p->norm_trace < 0 || p->norm_trace > SWIT_NORM_MAX) return set_err(SWIT_EINVAL, "bad norm_trace");
+    if (p->norm_event < 0 || p->norm_event > SWIT_NORM_L2) return set_err(SWIT_EINVAL, "bad norm_event");
+    if (p->mute_late_arrival && !(p->dt > 0.0)) return set_err(SWIT_EINVAL, "the window mute needs dt > 0");
+    return SWIT_OK;
+}
+
+size_t process_bytes(const swit_process_params *p) {
+    const size_t ntraces = (size_t)p->ngathers * p->ntr;
+    size_t b = 0;
+    if (p->filter_taps) b += sizeof(double) * (size_t)(p->nt + 6 * p->filter_taps) * std::min<size_t>(ntraces, kProcChunk);
+    if (p->norm_event) b = std::max(b, sizeof(float) * ntraces);
+    return std::max<size_t>(b, 256);
+}
+
+template <int K>
+void launch_filtfilt_k(const pr::ProcArgs &a, int n, float *data, double *y, hipStream_t st) {
+    hipLaunchKernelGGL(pr::k_filtfilt<K>, dim3((n + pr::kFilterThreads - 1) / pr::kFilterThreads),
+                       dim3(pr::kFilterThreads), 0, st, a, n, data, y);
+}
+
+template <int K = 2>
+void launch_filtfilt(int taps, const pr::ProcArgs &a, int n, float *data, double *y, hipStream_t st) {
+    if constexpr (K <= pr::kMaxTaps) {
+        if (taps == K) return launch_filtfilt_k<K>(a, n, data, y, st);
+        launch_filtfilt<K + 1>(taps, a, n, data, y, st);
+    }
+}
+}  // namespace
+
+extern "C" int swit_process_work(const swit_process_params *p, size_t *bytes) {
+    g_err[0] = 0;
+    if (const int rc = process_check(p)) return rc;
+    if (!bytes) return set_err(SWIT_EINVAL, "bytes is required");
+    *bytes = process_bytes(p);
+    return SWIT_OK;
+}
+
+extern "C" int swit_process(const swit_process_params *p, const float *offsets, const double *mute_win,
+                            const double *taper, float *data, void *work, size_t work_bytes, hipStream_t stream) {
+    g_err[0] = 0;
+    if (const int rc = process_check(p)) return rc;
+    if (!data || !work) return set_err(SWIT_EINVAL, "data and work are required");
+    if (work_bytes < process_bytes(p)) return set_err(SWIT_EINVAL, "work needs %zu bytes", process_bytes(p));
+    if ((p->mute_offset_short || p->mute_offset_long) && !offsets)
+        return set_err(SWIT_EINVAL, "offset mutes need offsets");
+    if (p->mute_late_arrival && !mute_win) return set_err(SWIT_EINVAL, "the window mute needs mute_win");
+    if (p->detrend_taper && !taper) return set_err(SWIT_EINVAL, "detrend_taper needs taper");
+    pr::ProcArgs a{};
+    a.ntr = p->ntr;
+    a.nt = p->nt;
+    a.mute_short = p->mute_offset_short;
+    a.mute_long = p->mute_offset_long;
+    a.short_dist = p->mute_offset_short_dis;
+    a.long_dist = p->mute_offset_long_dis;
+    a.mute_arrival = p->mute_late_arrival ? (p->mute_late_window > 0.0 ? 2 : 1) : 0;
+    a.pick_shift = p->mute_late_arrival ? std::ceil(std::fabs(p->mute_late_window) / p->dt) : 0.0;
+    a.taps = p->filter_taps;
+    a.pad = 3 * p->filter_taps;
+    for (int k = 0; k < p->filter_taps; ++k) {
+        a.b[k] = p->b[k];
+        a.a[k] = p->a[k];
+        a.zi[k] = k + 1 < p->filter_taps ? p->zi[k] : 0.0;
+    }
+    a.norm_trace = p->norm_trace;
+    a.norm_event = p->norm_event;
+    const int ntraces = p->ngathers * p->ntr;
+    if (a.mute_short || a.mute_long || a.mute_arrival) {
+        hipLaunchKernelGGL(pr::k_mute, dim3(ntraces), dim3(pr::kThreads), 0, stream, a, offsets, mute_win, data);
+        LAUNCH_CHECK();
+    }
+    if (a.taps) {
+        double *y = static_cast<double *>(work);
+        for (int t0 = 0; t0 < ntraces; t0 += kProcChunk) {
+            const int n = std::min(kProcChunk, ntraces - t0);
+            launch_filtfilt(a.taps, a, n, data + (size_t)t0 * p->nt, y, stream);
+            LAUNCH_CHECK();
+        }
+    }
+    if (p->detrend_taper) {
+        hipLaunchKernelGGL(pr::k_detrend_taper, dim3(ntraces), dim3(pr::kThreads), 0, stream, a, taper, data);
+        LAUNCH_CHECK();
+    }
+    if (a.norm_trace || a.norm_event) {
+        float *enorm = static_cast<float *>(work);
+        hipLaunchKernelGGL(pr::k_norm_trace, dim3(ntraces), dim3(pr::kThreads), 0, stream, a, data, enorm);
+        LAUNCH_CHECK();
+        if (a.norm_event) {
+            hipLaunchKernelGGL(pr::k_norm_event, dim3(p->ngathers), dim3(pr::kThreads), 0, stream, a, enorm, data);
+            LAUNCH_CHECK();
+        }
+    }
     return SWIT_OK;
 }

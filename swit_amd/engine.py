@@ -326,3 +326,77 @@ def grad_precond(grad, forw, back, vpmax, marine, grad_mute=0, grad_thred=0.0, g
                                         ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(work.data_ptr()),
                                         nbytes.value, stream))
     return out
+
+
+def process_params(optim, ngathers, ntr, nt, dt):
+    """swit_process parameters for `optim`'s processing (toolbox/preprocess.py:47-200)
+    and the tables it takes: (params, mute window [100] or None, taper [nt] or None)."""
+    from scipy.signal import lfilter_zi
+    from . import preprocess
+    p = _native.SwitProcessParams()
+    p.ngathers, p.ntr, p.nt, p.dt = int(ngathers), int(ntr), int(nt), float(dt)
+    p.mute_offset_short = int(bool(optim.mute_offset_short))
+    p.mute_offset_long = int(bool(optim.mute_offset_long))
+    p.mute_offset_short_dis = float(optim.mute_offset_short_dis)
+    p.mute_offset_long_dis = float(optim.mute_offset_long_dis)
+    p.mute_late_arrival = int(bool(optim.mute_late_arrival))
+    p.mute_late_window = float(optim.mute_late_window)
+    win = taper = None
+    if p.mute_late_arrival:
+        L = preprocess.MUTE_LENGTH
+        win = np.sin(np.linspace(0, np.pi, 2 * L))[0:L]
+    ba = preprocess.filter_design(optim, dt)
+    if ba is not None:
+        b, a = ba
+        if len(b) != len(a) or len(b) > _native.SWIT_PROCESS_MAX_TAPS or a[0] != 1.0:
+            raise ValueError("filter coefficients must have len(b) = len(a) <= %d and a[0] = 1"
+                             % _native.SWIT_PROCESS_MAX_TAPS)
+        zi = lfilter_zi(b, a)
+        p.filter_taps = len(b)
+        for k in range(len(b)):
+            p.b[k], p.a[k] = float(b[k]), float(a[k])
+        for k in range(len(zi)):
+            p.zi[k] = float(zi[k])
+        p.detrend_taper = 1
+        taper = preprocess.obspy_taper(nt)
+    nz = optim.normalize
+    p.norm_trace = (_native.SWIT_NORM["l1"] if 'L1-Trace' in nz else _native.SWIT_NORM["l2"] if 'L2-Trace' in nz
+                    else _native.SWIT_NORM["max"] if 'Max-Trace' in nz else 0)
+    p.norm_event = (_native.SWIT_NORM["l1"] if 'L1-Event' in nz else _native.SWIT_NORM["l2"] if 'L2-Event' in nz
+                    else 0)
+    return p, win, taper
+
+
+def process_gathers(optim, data, dt, offsets=None, params=None):
+    """process_workflow's steps (toolbox/preprocess.py:47-200) on device gathers, in
+    place: data float32 device tensor [ngathers][ntr][nt] (or [ntr][nt]); offsets
+    [ngathers][ntr] group x - source x (required with an offset mute).  params: the
+    (params, mute window, taper) of process_params, to override the derived ones.
+    Returns data."""
+    torch = _require_gpu()
+    lib = _native.load()
+    if data.dtype != torch.float32 or not data.is_cuda or not data.is_contiguous():
+        raise ValueError("data must be a contiguous float32 device tensor")
+    shp = data.shape if data.dim() == 3 else (1,) + tuple(data.shape)
+    ng, ntr, nt = (int(x) for x in shp)
+    p, win, taper = params if params is not None else process_params(optim, ng, ntr, nt, dt)
+    dev = data.device
+    off = None
+    if p.mute_offset_short or p.mute_offset_long:
+        if offsets is None:
+            raise ValueError("offset mutes need the trace offsets")
+        off = torch.as_tensor(np.asarray(offsets, np.float32).reshape(-1), device=dev)
+        if off.numel() != ng * ntr:
+            raise ValueError("offsets must hold one value per trace")
+    w = torch.as_tensor(win, dtype=torch.float64, device=dev) if win is not None else None
+    tp = torch.as_tensor(taper, dtype=torch.float64, device=dev) if taper is not None else None
+    nbytes = ctypes.c_size_t()
+    _native.check(lib.swit_process_work(ctypes.byref(p), ctypes.byref(nbytes)))
+    work = torch.empty(int(nbytes.value), dtype=torch.uint8, device=dev)
+    stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+    def ptr(t):
+        return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+    _native.check(lib.swit_process(ctypes.byref(p), ptr(off), ptr(w), ptr(tp), ptr(data), ptr(work), nbytes.value,
+                                   stream))
+    return data

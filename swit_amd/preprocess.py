@@ -3,11 +3,18 @@ data/<type>/src<i>_sg.su -> data/<type>/src<i>_sg_proc.su with offset/arrival mu
 Butterworth filtering (scipy butter/filtfilt, order 4) and normalisation, applied in
 the reference's order (mute, filter, normalize; preprocess.py:47-61).
 
+The reference works on obspy traces, whose data stay float32 throughout: the mute
+multiplies in place, the filtered float64 result is stored back as float32, and the
+detrend ('demean', then 'linear', both scipy.signal.detrend on float32 data) and the
+5 % Hann taper of obspy's Trace.taper follow the filter (preprocess.py:64-95).  This
+restatement keeps that float32 flow, trace by trace, with the same scipy calls.
+obspy itself is not available here: its detrend and taper are restated from their
+documented behaviour and their parity is unpinned.
+
 With no filter, mute or normalisation (the benchmark configurations) the processed
-file is a byte copy of the solver's SU file.  The reference's detrend/taper after
-filtering and first-break picking come from obspy, which is not available here: they
-are restated with scipy (demean, linear detrend, 5 % Hann taper) and their parity is
-unpinned.
+file is a byte copy of the solver's SU file.
+
+`process_gathers_device` runs the same steps on device gathers through swit_process.
 """
 from __future__ import annotations
 
@@ -15,8 +22,11 @@ import shutil
 
 import numpy as np
 from scipy.signal import butter, detrend, filtfilt
+from scipy.signal.windows import hann
 
 from . import su
+
+MUTE_LENGTH = 100        # samples of the arrival-mute taper (preprocess.py:101)
 
 
 def butter_bandpass_filter(data, lowcut, highcut, dt, order=4):
@@ -37,18 +47,35 @@ def butter_highpass_filter(data, lowcut, dt, order=4):
     return filtfilt(b, a, data)
 
 
-def _hann_taper(n, pct=0.05):
-    w = np.ones(n)
-    k = int(pct * n)
-    if k > 0:
-        ramp = 0.5 * (1 - np.cos(np.pi * np.arange(k) / k))
-        w[:k] = ramp
-        w[n - k:] = ramp[::-1]
-    return w
+def filter_design(optim, dt, order=4):
+    """(b, a) of the Butterworth filter apply_filter uses, or None for 'none'
+    (preprocess.py:75-81, :273-296): 'bandpass' fre_low..fre_high, 'lowpass' at
+    fre_low, 'highpass' at fre_high."""
+    kind = optim.fre_filter.lower()
+    nyq = 0.5 / dt
+    if kind == 'none':
+        return None
+    if kind == 'bandpass':
+        return butter(order, [optim.fre_low / nyq, optim.fre_high / nyq], btype='band')
+    if kind == 'lowpass':
+        return butter(order, optim.fre_low / nyq, btype='low')
+    if kind == 'highpass':
+        return butter(order, optim.fre_high / nyq, btype='hp')
+    raise ValueError(optim.fre_filter)
+
+
+def obspy_taper(npts, max_percentage=0.05):
+    """Weights of obspy's Trace.taper(max_percentage, type='hann'): half-length
+    min(int(p npts), int(npts / 2)), sides from scipy's symmetric Hann window of
+    2 half-length + 1 points (2 half-length when that equals npts), ones between."""
+    wlen = min(int(max_percentage * npts), int(npts / 2))
+    sides = hann(2 * wlen) if 2 * wlen == npts else hann(2 * wlen + 1)
+    return np.hstack((sides[:wlen], np.ones(npts - 2 * wlen), sides[len(sides) - wlen:]))
 
 
 def apply_filter(optim, data, dt):
-    """preprocess.py:64-95."""
+    """preprocess.py:64-95 on a float32 gather: filtfilt (float64) stored back as
+    float32, then per trace demean, linear detrend and the 5 % Hann taper."""
     kind = optim.fre_filter.lower()
     if kind == 'none':
         return data
@@ -60,8 +87,14 @@ def apply_filter(optim, data, dt):
         out = butter_highpass_filter(data, optim.fre_high, dt)
     else:
         raise ValueError(optim.fre_filter)
-    out = detrend(out - out.mean(axis=-1, keepdims=True), axis=-1, type='linear')
-    return out * _hann_taper(out.shape[-1])
+    out = out.astype(np.float32)
+    taper = obspy_taper(out.shape[-1])
+    for i in range(out.shape[0]):
+        tr = detrend(out[i], type='constant')
+        tr = detrend(tr, type='linear')
+        tr *= taper
+        out[i] = tr
+    return out
 
 
 def mask(itmin, itmax, nt, length):
@@ -88,9 +121,10 @@ def brutal_picker(data):
 
 
 def apply_mute(optim, data, dt, offsets):
-    """Offset mutes and first-break window mute (preprocess.py:98-151)."""
+    """Offset mutes, then the first-break window mute (preprocess.py:98-151), in place
+    on a copy of the float32 gather as the reference's in-place trace products."""
     data = data.copy()
-    length = 100
+    length = MUTE_LENGTH
     nt = data.shape[-1]
     off = np.abs(offsets)
     if optim.mute_offset_short:
@@ -109,23 +143,27 @@ def apply_mute(optim, data, dt, offsets):
 
 
 def apply_normalize(optim, data):
-    """preprocess.py:154-200."""
+    """preprocess.py:154-200 on a float32 gather, trace by trace as the reference (its
+    norms are float32 reductions; the event sum starts from 0. and adds them in trace
+    order)."""
     data = data.copy()
     nz = optim.normalize
-    if 'L1-Trace' in nz:
-        w = np.linalg.norm(data, ord=1, axis=-1)
-    elif 'L2-Trace' in nz:
-        w = np.linalg.norm(data, ord=2, axis=-1)
-    elif 'Max-Trace' in nz:
-        w = np.max(np.abs(data), axis=-1)
-    else:
-        w = None
-    if w is not None:
-        data[w > 0] /= w[w > 0, None]
-    if 'L1-Event' in nz:
-        data /= np.sum(np.linalg.norm(data, ord=1, axis=-1))
-    elif 'L2-Event' in nz:
-        data /= np.sum(np.linalg.norm(data, ord=2, axis=-1))
+    if 'L1-Trace' in nz or 'L2-Trace' in nz or 'Max-Trace' in nz:
+        for i in range(data.shape[0]):
+            if 'L1-Trace' in nz:
+                w = np.linalg.norm(data[i], ord=1)
+            elif 'L2-Trace' in nz:
+                w = np.linalg.norm(data[i], ord=2)
+            else:
+                w = np.max(abs(data[i]))
+            if w > 0:
+                data[i] /= w
+    if 'L1-Event' in nz or 'L2-Event' in nz:
+        w = 0.
+        for i in range(data.shape[0]):
+            w += np.linalg.norm(data[i], ord=1 if 'L1-Event' in nz else 2)
+        for i in range(data.shape[0]):
+            data[i] /= w
     return data
 
 
@@ -148,8 +186,15 @@ def process_workflow(simu, optim, simu_type='syn', use_first_break_in_su_header=
             continue
         data, hdr = su.read(load)
         dt = float(hdr["dt"][0]) * 1e-6
-        data = data.astype(np.float64)
-        data = apply_mute(optim, data, dt, hdr["offset"].astype(np.float64))
+        data = apply_mute(optim, data.astype(np.float32), dt, hdr["offset"].astype(np.float64))
         data = apply_filter(optim, data, dt)
         data = apply_normalize(optim, data)
         su.write(save, data.astype(np.float32), np.float32(hdr["d1"][0]), hdr["sx"][0], hdr["gx"])
+
+
+def process_gathers_device(optim, data, dt, offsets=None):
+    """apply_mute, apply_filter and apply_normalize on device gathers (float32 device
+    tensor [ngathers][ntr][nt] or [ntr][nt], processed in place) through swit_process;
+    offsets: group x - source x per trace."""
+    from . import engine
+    return engine.process_gathers(optim, data, dt, offsets)

@@ -1,0 +1,127 @@
+"""Host restatement of toolbox/preprocess.py (swit_amd.preprocess), pinned by closed
+forms: the mute mask's branches, the first-break pick, obspy's Hann taper, zero-phase
+filtering of sinusoids in and out of the band, the normalisations, and the
+direct-form II transposed recurrence the device filter uses, against scipy's lfilter
+bit for bit."""
+import numpy as np
+import pytest
+
+from swit_amd import base, preprocess
+
+
+def _optim(**kw):
+    o = base.optimize('Waveform', 'LBFGS', 1, 0.05, 6000.0, 1000.0, 'Marine', 0, 0, 'None', 0.0, 0.0, 0, 0.0,
+                      ['None'], 0, 0, 0.0, 0.0)
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def test_mask_branches():
+    L = 100
+    win = np.sin(np.linspace(0, np.pi, 2 * L))[:L]
+    m = preprocess.mask(300, 400, 1000, L)                 # 1 < itmin < itmax < nt
+    assert np.all(m[:300] == 0) and np.array_equal(m[300:400], win) and np.all(m[400:] == 1)
+    m = preprocess.mask(-40, 60, 1000, L)                  # itmin < 1 <= itmax
+    assert np.array_equal(m[:60], win[40:]) and np.all(m[60:] == 1)
+    m = preprocess.mask(950, 1050, 1000, L)                # itmin < nt < itmax
+    assert np.all(m[:950] == 0) and np.array_equal(m[950:], win[:50])
+    assert np.all(preprocess.mask(1200, 1300, 1000, L) == 0)   # itmin > nt
+    assert np.all(preprocess.mask(1, 101, 1000, L) == 1)       # no branch: itmin == 1
+
+
+def test_brutal_picker_and_window_mute():
+    nt, dt = 600, 0.002
+    d = np.zeros((3, nt), np.float32)
+    d[0, 200:] = 1.0
+    d[1, 350] = -2.0
+    d[1, 349] = 0.001                                      # below 0.1 % of the trace maximum
+    d[2] = 0.0
+    pick = preprocess.brutal_picker(d)
+    assert list(pick) == [200, 350, 0]
+    o = _optim(mute_late_arrival=1, mute_late_window=0.1)  # late: keep before pick + 50 samples
+    out = preprocess.apply_mute(o, d, dt, np.zeros(3))
+    # pick + ceil(0.1 / 0.002) = pick + 50; window [pick + 0, pick + 100) tapers 1 -> 0
+    assert out[0, 199] == 0 and out[0, 200] == np.float32(1.0 - preprocess.mask(200, 300, nt, 100)[200])
+    assert np.all(out[0, 300:] == 0)
+    o = _optim(mute_late_arrival=1, mute_late_window=-0.1)  # early: zero before the window
+    out = preprocess.apply_mute(o, d, dt, np.zeros(3))
+    # window [350, 450): zero before it, rising sine taper inside
+    assert np.all(out[1, :350] == 0) and np.all(out[1, 351:] == 0)
+    assert np.all(out[0, :200] == 0) and out[0, 250] == np.float32(preprocess.mask(200, 300, nt, 100)[250])
+    assert np.all(out[0, 300:] == 1)
+
+
+def test_offset_mutes():
+    d = np.ones((4, 50), np.float32)
+    o = _optim(mute_offset_short=1, mute_offset_short_dis=100.0, mute_offset_long=1, mute_offset_long_dis=1000.0)
+    out = preprocess.apply_mute(o, d, 0.002, np.array([-50.0, 100.0, -999.0, 1001.0]))
+    assert [bool(np.all(r == 0)) for r in out] == [True, False, False, True]
+
+
+def test_obspy_taper():
+    t = preprocess.obspy_taper(1000)
+    k = np.arange(50)
+    np.testing.assert_allclose(t[:50], 0.5 - 0.5 * np.cos(np.pi * k / 50), atol=1e-15)
+    assert np.all(t[50:950] == 1.0)
+    np.testing.assert_allclose(t[950:], t[:50][::-1], atol=1e-15)
+    assert preprocess.obspy_taper(10).size == 10 and np.all(preprocess.obspy_taper(10) == 1.0)
+
+
+@pytest.mark.parametrize("kind,f_in,f_out", [("bandpass", 10.0, 60.0), ("lowpass", 5.0, 40.0),
+                                             ("highpass", 30.0, 2.0)])
+def test_filter_passes_band_and_stops_the_rest(kind, f_in, f_out):
+    """Zero-phase Butterworth: a sinusoid inside the band comes out as the input minus
+    its mean and least-squares line (the detrend after the filter), with no phase
+    shift, away from the taper; one far outside is attenuated.  'lowpass' cuts at
+    fre_low and 'highpass' at fre_high, as preprocess.py:75-81 passes them."""
+    from scipy.signal import detrend
+    nt, dt = 2000, 0.002
+    t = np.arange(nt) * dt
+    o = _optim(fre_filter=kind, fre_low=20.0 if kind != 'bandpass' else 5.0, fre_high=15.0 if kind != 'bandpass'
+               else 20.0)
+    if kind == 'lowpass':
+        o.fre_low = 20.0
+    if kind == 'highpass':
+        o.fre_high = 15.0
+    d = np.array([np.sin(2 * np.pi * f_in * t), np.sin(2 * np.pi * f_out * t)], np.float32)
+    out = preprocess.apply_filter(o, d, dt)
+    mid = slice(300, 1700)
+    assert out.dtype == np.float32
+    expect = detrend(detrend(d[0].astype(np.float64), type='constant'), type='linear')
+    np.testing.assert_allclose(out[0, mid], expect[mid], atol=1e-2)
+    assert np.abs(out[1, mid]).max() < 2e-2
+
+
+def test_normalisations():
+    rng = np.random.default_rng(0)
+    d = rng.standard_normal((5, 300)).astype(np.float32)
+    d[2] = 0.0
+    out = preprocess.apply_normalize(_optim(normalize=['Max-Trace']), d)
+    assert np.allclose(np.abs(out[[0, 1, 3, 4]]).max(-1), 1.0) and np.all(out[2] == 0)
+    out = preprocess.apply_normalize(_optim(normalize=['L2-Trace']), d)
+    assert np.allclose(np.linalg.norm(out[[0, 1, 3, 4]], axis=-1), 1.0, rtol=1e-6)
+    out = preprocess.apply_normalize(_optim(normalize=['L1-Event']), d)
+    assert np.sum(np.abs(out)) == pytest.approx(1.0, rel=1e-5)
+
+
+def test_df2t_order_is_scipys():
+    """The recurrence the device filter runs, y = z0 + b0 x, z_k = z_(k+1) + x b_(k+1)
+    - y a_(k+1), reproduces scipy.signal.lfilter bit for bit (outputs and final state);
+    other orders of the same sums differ in the last bits."""
+    from scipy.signal import butter, lfilter, lfilter_zi
+    b, a = butter(4, [3 / 250, 20 / 250], btype='band')
+    x = np.random.default_rng(1).standard_normal(400)
+    zi = lfilter_zi(b, a) * 0.7
+    y_ref, zf_ref = lfilter(b, a, x, zi=zi)
+    z = zi.copy()
+    y = np.zeros_like(x)
+    K = len(b)
+    for n in range(len(x)):
+        xn = x[n]
+        yn = z[0] + b[0] * xn
+        for k in range(K - 2):
+            z[k] = z[k + 1] + xn * b[k + 1] - yn * a[k + 1]
+        z[K - 2] = xn * b[K - 1] - yn * a[K - 1]
+        y[n] = yn
+    assert np.array_equal(y, y_ref) and np.array_equal(z, zf_ref)
