@@ -117,3 +117,48 @@ def test_inversion_loop_reduces_misfit(tmp_path, scheme):
     print(scheme, f)
     assert len(f) == 4 and np.all(np.diff(f) < 0)
     solver.release()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,filt,norm", [("Waveform", "None", "None"),
+                                            ("Globalcorrelation", "Bandpass", ["L2-Trace"])])
+def test_device_loop_matches_file_loop(tmp_path, kind, filt, norm):
+    """The device-resident loop (swit_amd.inversion_device: device processing, misfit,
+    adjoint source, preconditioning and L-BFGS vectors, no SU files) against the
+    file-based loop on the drop-in solver over three L-BFGS iterations: the same
+    line-search decisions, misfits to 1e-4 (the logs keep 5 digits) and models to
+    float32 rounding of the different inner products and processing."""
+    from swit_amd import base, inversion, inversion_device, preprocess, solver
+    from swit_amd.tools import loadbinfloat32
+    w = workloads.small(seed=24, nx=60, nz=36, npml=10, nt=300, ns=3)
+    w.rec_xz = w.rec_xz[:, :w.nx]
+
+    def optim():
+        return base.optimize(kind, 'LBFGS', 3, 0.05, 3500.0, 1400.0, 'Marine', 3, 2, filt, 3.0, 25.0,
+                             False, 0.0, norm, False, False, 0.0, 1e9)
+    init = {'vp': w.vp_init.astype(np.float64), 'rho': workloads.gardner(w.vp_init).astype(np.float64)}
+    home_a = str(tmp_path / 'a') + "/"
+    os.makedirs(home_a)
+    simu_true = _simu(home_a, w, w.vp_true)
+    o = optim()
+    solver.forward(simu_true, 'obs', savesnap=0)
+    preprocess.process_workflow(simu_true, o, 'obs')
+    inversion.inversion(_simu(home_a, w, w.vp_init), o, dict(init))
+    solver.release()
+
+    home_b = str(tmp_path / 'b') + "/"
+    os.makedirs(home_b)
+    simu = _simu(home_b, w, w.vp_init)
+    o = optim()
+    inversion_device.inversion(simu, o, dict(init), simu_true=_simu(home_b, w, w.vp_true))
+    fa = np.loadtxt(home_a + 'outputs/misfit_data.dat')
+    fb = np.loadtxt(home_b + 'outputs/misfit_data.dat')
+    print(kind, fa, fb)
+    np.testing.assert_array_equal(np.loadtxt(home_a + 'outputs/line_search_iteration.dat'),
+                                  np.loadtxt(home_b + 'outputs/line_search_iteration.dat'))
+    np.testing.assert_allclose(fb, fa, rtol=1e-4)
+    va = loadbinfloat32(home_a + 'outputs/velocity/vp-3.bin')
+    vb = loadbinfloat32(home_b + 'outputs/velocity/vp-3.bin')
+    np.testing.assert_allclose(vb, va, rtol=1e-4)
+    # a failed line search (an L-BFGS restart) keeps the misfit; none increases it
+    assert np.all(np.diff(fb) <= 0) and fb[-1] < fb[0]

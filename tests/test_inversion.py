@@ -94,3 +94,66 @@ def test_inversion_loop_on_quadratic(tmp_path, monkeypatch, scheme):
     for i in range(1, 13):
         for sub, stem in (('gradient', 'grad'), ('direction', 'dirc'), ('velocity', 'vp')):
             assert os.path.exists(home + 'outputs/%s/%s-%d.bin' % (sub, stem, i))
+
+
+@pytest.mark.parametrize("scheme", ["LBFGS", "NLCG"])
+def test_device_loop_matches_file_loop(tmp_path, monkeypatch, scheme):
+    """The device-resident loop (swit_amd.inversion_device), run on CPU tensors with the
+    same toy problem, takes the same decisions as the file-based loop: identical
+    line-search iteration counts and steps, misfits and models to float32 rounding
+    (its inner products are FP64, the file loop's L-BFGS history products float32)."""
+    import torch
+    from swit_amd import inversion_device as dinv
+    from swit_amd.tools import loadbinfloat32
+    maxiter = 10
+    _, _, home_a, _ = _toy(tmp_path / 'a', scheme, monkeypatch, maxiter)
+    nx, nz = 6, 5
+    rng = np.random.default_rng(3)
+    target = torch.from_numpy(rng.uniform(1800, 3200, (nx, nz)).ravel())
+    w = torch.from_numpy(rng.uniform(0.5, 4.0, (nx, nz)).ravel())
+
+    class Problem:
+        device = torch.device('cpu')
+
+        def __init__(self):
+            self.nx, self.nz = nx, nz
+
+        def misfit(self, v):
+            r = v - target
+            return 0.5 * float(torch.sum(w * r * r))
+
+        def gradient(self, v):
+            return self.misfit(v), w * (v - target)
+
+    home_b = str(tmp_path / 'b') + '/'
+    simu = types.SimpleNamespace(model=types.SimpleNamespace(nx=nx, nz=nz, vp=None, rho=None),
+                                 system=types.SimpleNamespace(homepath=home_b))
+    optim = types.SimpleNamespace(scheme=scheme, iter=0, maxiter=maxiter, misfit_type='Waveform',
+                                  step_length=0.05, vpmax=3500.0, vpmin=1400.0)
+    vp0 = np.full((nx, nz), 2500.0)
+    dinv.inversion(simu, optim, {'vp': vp0, 'rho': np.ones_like(vp0)}, problem=Problem())
+    for name in ('line_search_iteration.dat',):
+        np.testing.assert_array_equal(np.loadtxt(home_a + 'outputs/' + name), np.loadtxt(home_b + 'outputs/' + name))
+    np.testing.assert_allclose(np.loadtxt(home_a + 'outputs/line_search_step.dat'),
+                               np.loadtxt(home_b + 'outputs/line_search_step.dat'), rtol=1e-4)
+    np.testing.assert_allclose(np.loadtxt(home_a + 'outputs/misfit_data.dat'),
+                               np.loadtxt(home_b + 'outputs/misfit_data.dat'), rtol=2e-4)
+    for i in range(1, maxiter + 1):
+        va = loadbinfloat32(home_a + 'outputs/velocity/vp-%d.bin' % i)
+        vb = loadbinfloat32(home_b + 'outputs/velocity/vp-%d.bin' % i)
+        np.testing.assert_allclose(vb, va, rtol=1e-5)
+    np.testing.assert_allclose(simu.model.vp.ravel(), va, rtol=1e-5)
+
+
+def test_device_lbfgs_direction_matches_host():
+    import torch
+    from swit_amd import inversion_device as dinv
+    rng = np.random.default_rng(5)
+    n, k = 40, 5
+    S = rng.standard_normal((n, k)).astype(np.float32)
+    Y = (S * rng.uniform(0.5, 2.0, (n, 1))).astype(np.float32)
+    g = rng.standard_normal(n)
+    zh = inv.lbfgs_direction(S, Y, k, g)
+    zd = dinv.lbfgs_direction(torch.from_numpy(np.ascontiguousarray(S.T)), torch.from_numpy(np.ascontiguousarray(Y.T)),
+                              k, torch.from_numpy(g))
+    np.testing.assert_allclose(zd.numpy(), zh, rtol=1e-5, atol=1e-6 * np.abs(zh).max())
