@@ -210,6 +210,9 @@ def main():
                          "group with the global-correlation misfit on the device; lbfgs: --iters "
                          "L-BFGS iterations of the reference inversion loop through the drop-in solver "
                          "on the 801x267x6000, 81-shot model with 16-bit strips (--strips, default f16)")
+    ap.add_argument("--resident", action="store_true",
+                    help="with --workload lbfgs: run the device-resident loop (swit_amd.inversion_device) "
+                         "instead of the file-based one")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (5 times): the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -396,29 +399,44 @@ def bench_lbfgs(args):
     optim = base.optimize('Waveform', 'LBFGS', args.iters, 0.05, 6000.0, 2000.0, 'Marine', 5, 0, 'None',
                           1.0, 10.0, False, 0.0, 'None', False, False, 0.0, 1e9)
     solver.STRIPS = strips
-    t0 = time.perf_counter()
-    simu_true = simulate(w.vp_true)
-    solver.forward(simu_true, 'obs', savesnap=0)
-    preprocess.process_workflow(simu_true, optim, 'obs')
-    t_obs = time.perf_counter() - t0
-    simu = simulate(w.vp_init)
-    t0 = time.perf_counter()
-    inversion.inversion(simu, optim, {'vp': w.vp_init.astype(np.float64),
-                                      'rho': w.rho(w.vp_init).astype(np.float64)})
-    el = time.perf_counter() - t0
+    init = {'vp': w.vp_init.astype(np.float64), 'rho': w.rho(w.vp_init).astype(np.float64)}
+    if args.resident:
+        # device-resident loop: observed data modelled and kept on the device
+        from swit_amd import inversion_device
+        t0 = time.perf_counter()
+        problem = inversion_device.DeviceProblem(simulate(w.vp_init), optim, simu_true=simulate(w.vp_true))
+        t_obs = time.perf_counter() - t0
+        simu = simulate(w.vp_init)
+        t0 = time.perf_counter()
+        inversion_device.inversion(simu, optim, init, problem=problem)
+        el = time.perf_counter() - t0
+        problem.close()
+    else:
+        t0 = time.perf_counter()
+        simu_true = simulate(w.vp_true)
+        solver.forward(simu_true, 'obs', savesnap=0)
+        preprocess.process_workflow(simu_true, optim, 'obs')
+        t_obs = time.perf_counter() - t0
+        simu = simulate(w.vp_init)
+        t0 = time.perf_counter()
+        inversion.inversion(simu, optim, init)
+        el = time.perf_counter() - t0
     f = np.loadtxt(home + 'outputs/misfit_data.dat')
     steps = np.loadtxt(home + 'outputs/line_search_iteration.dat').reshape(-1)
     solver.release()
     shutil.rmtree(home, ignore_errors=True)
 
     line = {
-        "metric": "L-BFGS iterations through the reference inversion loop, 801x267x6000, 81 shots",
+        "metric": "L-BFGS iterations through the reference inversion loop, 801x267x6000, 81 shots"
+                  + (", device-resident vectors" if args.resident else ""),
         "value": round(args.iters / el * 60.0, 3), "unit": "iterations/min", "n_gpus": 1,
         "steps": args.iters, "warmup": 0, "ms_per_step": round(1000 * el / args.iters, 1),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded folded layers 2000-6000 m/s, Ricker 5 Hz)",
         "config": {"workload": f"{w.nx}x{w.nz} dx {w.dx} npml {w.npml} nt {w.nt} dt {w.dt}, {w.ns} shots, "
-                               f"{w.ng} receivers, waveform misfit, drop-in solver with SU files",
+                               f"{w.ng} receivers, waveform misfit, "
+                               + ("device-resident loop (no SU files)" if args.resident
+                                  else "drop-in solver with SU files"),
                    "strips": strips, "cross_shot_sum": "f64"},
         "misfit": [float(x) for x in f], "line_search_forwards": [int(x) for x in steps],
         "observed_data_s": round(t_obs, 2),

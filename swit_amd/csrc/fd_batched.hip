@@ -185,16 +185,32 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
-// Encode the last step's strip row from the FP32 staging row (the forward step kernel
-// encodes each earlier row one step later).  grid (ceil(S/256), ns).
+// Encode strip rows first_row .. first_row + gridDim.x - 1 of every shot (grid (rows, ns),
+// kEncThreads threads) from the FP32 ring ring[ns][nring][S] (row r in slot r % nring):
+// the row's max |v| goes to smax (float bits) and each value to 16 bits at that scale.
+constexpr int kEncThreads = 256;
+constexpr int kStripRing = 32;     // rows of the FP32 ring; encoded in batches of up to this many
 template <int SM>
-__global__ void k_strip_flush(Dims d, int row, const float *__restrict__ stage, const uint32_t *__restrict__ smax,
-                              uint16_t *__restrict__ strips16) {
-    const int s = blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.S) return;
+__global__ __launch_bounds__(kEncThreads) void k_strip_encode(Dims d, int first_row, int nring,
+                                                              const float *__restrict__ ring,
+                                                              uint32_t *__restrict__ smax,
+                                                              uint16_t *__restrict__ strips16) {
+    __shared__ float sh[kEncThreads / 64];
+    const int s = blockIdx.y, row = first_row + blockIdx.x;
+    const float *v = ring + ((size_t)s * nring + row % nring) * d.S;
+    float m = 0.0f;
+    for (int i = threadIdx.x; i < d.S; i += kEncThreads) m = fmaxf(m, fabsf(v[i]));
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < kEncThreads / 64; ++w) m = fmaxf(m, sh[w]);
     const size_t r = (size_t)s * d.nt + row;
-    strips16[r * d.S + i] = strip_encode<SM>(stage[(size_t)s * d.S + i] * strip_inv(smax[r]));
+    const uint32_t mb = __float_as_uint(m);
+    if (threadIdx.x == 0) smax[r] = mb;
+    const float inv = strip_inv(mb);
+    uint16_t *out = strips16 + r * d.S;
+    for (int i = threadIdx.x; i < d.S; i += kEncThreads) out[i] = strip_encode<SM>(v[i] * inv);
 }
 
 // Decode 16-bit strip rows to FP32: n = ns * nt * S values, row r = index / S.
@@ -268,15 +284,15 @@ struct DSave {
     int slots, base;
 };
 
-// strips: null (no saving), FP32 rows (SM = 0) or 16-bit rows (SM = 1, 2) with the
-// FP32 staging rows stage[2][ns][S] and per-row maxima smax[ns][nt] (float bits).
-template <int SM>
+// strips: null (no saving) or FP32 rows; step it's row of shot s is
+// strips[(s * s_rows + s_row) * S] (the plan's strips with s_rows = nt, s_row = it - 1, or
+// a ring of rows that k_strip_encode turns into 16-bit rows).
 __global__ __launch_bounds__(kThreads) void k_fwd_fused(
     Dims d, int it, TileMap tm, int ns, int G, const float2 *__restrict__ CP, const float4 *__restrict__ CV,
     const ShotGeom *__restrict__ geo, const float *__restrict__ src, const float *__restrict__ In,
     float *__restrict__ Out, const int *__restrict__ rec_start, const RecRef *__restrict__ rec_list,
     float *__restrict__ seis_t, const int *__restrict__ strip_start, const StripRef *__restrict__ strip_list,
-    void *__restrict__ strips, float *__restrict__ stage, uint32_t *__restrict__ smax, DSave dsave) {
+    float *__restrict__ strips, int s_rows, int s_row, DSave dsave) {
     using namespace fk;
     __shared__ float sP[kNR];
     __shared__ float sU[kNR];
@@ -426,33 +442,11 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
         // strips (fdcore.f90:130-159)
         if (strips) {
             int a0 = strip_start[t], a1 = strip_start[t + 1];
-            if constexpr (SM == 0) {
-                float *row = static_cast<float *>(strips) + ((size_t)s * d.nt + (it - 1)) * d.S;
-                for (int a = a0 + tid; a < a1; a += kThreads) {
-                    StripRef sr = strip_list[a];
-                    int l = L(sr.iz, sr.ix, zo, xo);
-                    row[sr.slot] = sr.field == 0 ? sQ[l] : (sr.field == 1 ? sU[l] : sW[l]);
-                }
-            } else if (a1 > a0) {
-                // stage this step's values, encode the previous step's (its max is final)
-                float *now = stage + ((size_t)(it & 1) * ns + s) * d.S;
-                const float *prev = stage + ((size_t)((it - 1) & 1) * ns + s) * d.S;
-                const bool enc = it >= 3;
-                const size_t rprev = (size_t)s * d.nt + (it - 2);
-                uint16_t *row16 = static_cast<uint16_t *>(strips) + rprev * d.S;
-                const float inv = enc ? strip_inv(smax[rprev]) : 0.0f;
-                float amax = 0.0f;
-                for (int a = a0 + tid; a < a1; a += kThreads) {
-                    StripRef sr = strip_list[a];
-                    int l = L(sr.iz, sr.ix, zo, xo);
-                    float v = sr.field == 0 ? sQ[l] : (sr.field == 1 ? sU[l] : sW[l]);
-                    now[sr.slot] = v;
-                    amax = fmaxf(amax, fabsf(v));
-                    if (enc) row16[sr.slot] = strip_encode<SM>(prev[sr.slot] * inv);
-                }
-                amax = wave_max(amax);
-                if ((tid & 63) == 0 && amax > 0.0f)
-                    atomicMax(&smax[(size_t)s * d.nt + (it - 1)], __float_as_uint(amax));
+            float *row = strips + ((size_t)s * s_rows + s_row) * d.S;
+            for (int a = a0 + tid; a < a1; a += kThreads) {
+                StripRef sr = strip_list[a];
+                int l = L(sr.iz, sr.ix, zo, xo);
+                row[sr.slot] = sr.field == 0 ? sQ[l] : (sr.field == 1 ? sU[l] : sW[l]);
             }
         }
         __syncthreads();
@@ -1128,7 +1122,7 @@ struct BatchedState {
     int strip_mode = 0;          // 0 FP32, 1 FP16, 2 BF16
     uint16_t *strips16 = nullptr;   // [ns][nt][S] 16-bit strips
     uint32_t *smax = nullptr;    // [ns][nt] per-row max |v| (float bits)
-    float *stage = nullptr;      // [2][ns][S] FP32 staging rows
+    float *ring = nullptr;       // [ns][kStripRing][S] FP32 rows awaiting their 16-bit encoding
     double *out = nullptr;       // [3][nx][nz]
     hipStream_t cap = nullptr;   // capture stream
     hipGraphExec_t gfwd[2] = {nullptr, nullptr};

@@ -678,7 +678,7 @@ static int batched_init(swit_plan *pl) {
     if (bs.strip_mode) {
         TRY(pl->alloc(&bs.strips16, (size_t)ns * d.nt * d.S));
         TRY(pl->alloc(&bs.smax, (size_t)ns * d.nt));
-        TRY(pl->alloc(&bs.stage, (size_t)2 * ns * d.S));
+        TRY(pl->alloc(&bs.ring, (size_t)kStripRing * ns * d.S));
     }
     HIP_TRY(hipMemcpy(bs.rec_start, rs.data(), sizeof(int) * rs.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(bs.rec_list, rl.data(), sizeof(RecRef) * rl.size(), hipMemcpyHostToDevice));
@@ -778,15 +778,30 @@ static int forward_prologue(swit_plan *pl, int save, hipStream_t st) {
     return SWIT_OK;
 }
 
-template <int SM>
+// Forward step it.  With strip saving the step's FP32 strip row goes to the plan's
+// strips (FP32 plans) or to the ring that k_strip_encode empties (16-bit plans).
 static void launch_fwd(swit_plan *pl, int it, const float *in, float *out, int save, hipStream_t st,
                        DSave dsave = DSave{nullptr, 0, 0}) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
-    void *strips = !save ? nullptr : SM == 0 ? static_cast<void *>(pl->strips) : static_cast<void *>(bs.strips16);
-    hipLaunchKernelGGL(k_fwd_fused<SM>, dim3(bs.ngroups_f * bs.fwd.ntiles), dim3(kThreads), 0, st, d, it, bs.fwd,
+    float *strips = !save ? nullptr : bs.strip_mode == 0 ? pl->strips : bs.ring;
+    const int s_rows = bs.strip_mode == 0 ? d.nt : kStripRing;
+    const int s_row = bs.strip_mode == 0 ? it - 1 : (it - 1) % kStripRing;
+    hipLaunchKernelGGL(k_fwd_fused, dim3(bs.ngroups_f * bs.fwd.ntiles), dim3(kThreads), 0, st, d, it, bs.fwd,
                        pl->ns, bs.Gf, bs.CP, bs.CV, pl->geo, bs.src, in, out, bs.rec_start, bs.rec_list,
-                       pl->seis_t, bs.strip_start, bs.strip_list, strips, bs.stage, bs.smax, dsave);
+                       pl->seis_t, bs.strip_start, bs.strip_list, strips, s_rows, s_row, dsave);
+}
+
+// 16-bit plans: encode strip rows first..last (at most kStripRing, all still in the ring).
+static void launch_encode(swit_plan *pl, int first, int last, hipStream_t st) {
+    BatchedState &bs = pl->bat;
+    const dim3 grd(last - first + 1, pl->ns);
+    if (bs.strip_mode == 1)
+        hipLaunchKernelGGL(k_strip_encode<1>, grd, dim3(kEncThreads), 0, st, pl->d, first, kStripRing, bs.ring,
+                           bs.smax, bs.strips16);
+    else
+        hipLaunchKernelGGL(k_strip_encode<2>, grd, dim3(kEncThreads), 0, st, pl->d, first, kStripRing, bs.ring,
+                           bs.smax, bs.strips16);
 }
 
 template <int SM>
@@ -829,15 +844,18 @@ static int launch_backward(swit_plan *pl, int it, const float *ri, float *ro, co
 static int forward_loop(swit_plan *pl, int save, hipStream_t st) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
+    int first = 1;                               // oldest strip row not yet encoded
     for (int it = 2; it <= d.nt; ++it) {
         const float *in = bs.F[it % 2];          // it = 2 reads F[0]
         float *out = bs.F[(it + 1) % 2];
-        switch (bs.strip_mode) {
-        case 0: launch_fwd<0>(pl, it, in, out, save, st); break;
-        case 1: launch_fwd<1>(pl, it, in, out, save, st); break;
-        default: launch_fwd<2>(pl, it, in, out, save, st); break;
-        }
+        launch_fwd(pl, it, in, out, save, st);
         LAUNCH_CHECK();
+        // 16-bit plans: encode the ring once its last slot is written, and at the end
+        if (save && bs.strip_mode && ((it - 1) % kStripRing == kStripRing - 1 || it == d.nt)) {
+            launch_encode(pl, first, it - 1, st);
+            LAUNCH_CHECK();
+            first = it;
+        }
     }
     return SWIT_OK;
 }
@@ -850,17 +868,6 @@ static int forward_epilogue(swit_plan *pl, int save, hipStream_t st) {
         hipLaunchKernelGGL(k_end_state, dim3(cdiv(d.nz, 64), cdiv(d.nx, 4), pl->ns), dim3(64, 4), 0, st, d, fin,
                            bs.end);
         LAUNCH_CHECK();
-        if (bs.strip_mode) {
-            // the last row (step nt) is still staged; the step kernels encoded rows 1..nt-2
-            // one step late and row 0 (step 1) is zero
-            const float *stage = bs.stage + (size_t)(d.nt & 1) * pl->ns * d.S;
-            dim3 grd(cdiv(d.S, 256), pl->ns);
-            if (bs.strip_mode == 1)
-                hipLaunchKernelGGL(k_strip_flush<1>, grd, dim3(256), 0, st, d, d.nt - 1, stage, bs.smax, bs.strips16);
-            else
-                hipLaunchKernelGGL(k_strip_flush<2>, grd, dim3(256), 0, st, d, d.nt - 1, stage, bs.smax, bs.strips16);
-            LAUNCH_CHECK();
-        }
     }
     return SWIT_OK;
 }
@@ -992,11 +999,8 @@ static int time_steps(swit_plan *pl, int which, int steps, hipStream_t st, doubl
     auto launch = [&](int k) {
         if (which == 0) {
             const int it = 2 + k % (d.nt - 1);
-            switch (bs.strip_mode) {   // with strip saving, as the gradient's forward runs
-            case 0: launch_fwd<0>(pl, it, bs.F[k & 1], bs.F[(k + 1) & 1], 1, st); break;
-            case 1: launch_fwd<1>(pl, it, bs.F[k & 1], bs.F[(k + 1) & 1], 1, st); break;
-            default: launch_fwd<2>(pl, it, bs.F[k & 1], bs.F[(k + 1) & 1], 1, st); break;
-            }
+            // with strip saving, as the gradient's forward runs
+            launch_fwd(pl, it, bs.F[k & 1], bs.F[(k + 1) & 1], 1, st);
         } else {
             // the plan's backward kernel; a two-step launch starts at it >= 3
             const int it = spl == 2 ? d.nt - (2 * k) % (d.nt - 2) : d.nt - k % (d.nt - 1);
@@ -1395,11 +1399,7 @@ static int exact_forward_steps(swit_plan *pl, int a, int b, DSave ds, hipStream_
     for (int it = a + 1; it <= b; ++it) {
         const float *in = bs.F[it % 2];
         float *out = bs.F[(it + 1) % 2];
-        switch (bs.strip_mode) {
-        case 0: launch_fwd<0>(pl, it, in, out, 0, st, ds); break;
-        case 1: launch_fwd<1>(pl, it, in, out, 0, st, ds); break;
-        default: launch_fwd<2>(pl, it, in, out, 0, st, ds); break;
-        }
+        launch_fwd(pl, it, in, out, 0, st, ds);
         LAUNCH_CHECK();
     }
     return SWIT_OK;

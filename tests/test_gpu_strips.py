@@ -35,7 +35,7 @@ def test_quantize_strips_closed_forms():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["f16", "bf16"])
-@pytest.mark.parametrize("nt", [150, 2, 3])
+@pytest.mark.parametrize("nt", [150, 2, 3, 33, 65])
 def test_16bit_strips_match_encoded_oracle(mode, nt):
     import torch
     from swit_amd.engine import Plan
