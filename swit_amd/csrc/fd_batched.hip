@@ -67,10 +67,11 @@ __device__ __forceinline__ int LW(int c, int l) { return c * kZL + l; }
 
 // Tiling of the padded grid: tile (zt, xt) covers rows
 // [max(ztop + zt*th, 1), min(ztop + (zt+1)*th - 1, nzp)] and columns
-// [1 + xt*TX, min((xt+1)*TX, nxp)].
+// [1 + xt*tx, min((xt+1)*tx, nxp)] (tx = kTX for the step kernels).
 struct TileMap {
     int ztop;
     int th;         // tile height (rows)
+    int tx;         // tile width (columns)
     int nzt, nxt;
     int ntiles;     // nzt * nxt
     int cg;         // tile columns per column group of the work order (see work_item)
@@ -85,8 +86,8 @@ __host__ __device__ inline TileBox tile_box(const Dims &d, const TileMap &tm, in
     TileBox b;
     b.z0 = max(tm.ztop + zt * tm.th, 1);
     b.z1 = min(tm.ztop + (zt + 1) * tm.th - 1, d.nzp);
-    b.x0 = 1 + xt * kTX;
-    b.x1 = min((xt + 1) * kTX, d.nxp);
+    b.x0 = 1 + xt * tm.tx;
+    b.x1 = min((xt + 1) * tm.tx, d.nxp);
     return b;
 }
 
@@ -1083,6 +1084,7 @@ struct BatchedState {
     // two-step backward kernel: its tiling, the injection groups of its tiles (step 2)
     // and of its tiles grown by 3 cells per side (step 1, per-shot capacity injx_cap)
     TileMap bwd2{};
+    TileMap extm{};              // tiles of the fused exact-adjoint step (ex::kExTX wide)
     int bwd_steps = 2;           // backward steps per launch: 1 or 2
     int *inj2_start = nullptr;
     InjGroup *inj2_groups = nullptr;

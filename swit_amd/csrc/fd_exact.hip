@@ -127,12 +127,18 @@ __global__ void k_ex_vel2(Dims d, const float *__restrict__ q, float *__restrict
     lw[k] = lw[k] - dz;
 }
 
+// Tile width of the fused step (its own tiling of the backward rows), window columns
+// (tile + 3-cell halo on each side) and floats per LDS array.
+constexpr int kExTX = 32;
+constexpr int kExCX = kExTX + 2 * kHalo;
+constexpr int kExNL = bk::kZL * kExCX;
+
 // ---------------------------------------------------------------------------------
 // One reverse step it of all shots, fused: the operations of k_ex_inject, k_ex_mirror
 // (w), k_ex_vel, k_ex_p, k_ex_mirror (p), k_ex_grad and k_ex_vel2, in that order, with
 // the same arithmetic per cell, so the same bits (receivers sharing a cell are added in
-// receiver order).  One workgroup per tile of the backward tiling (58x16) of one shot.
-// Its window is the tile plus 3 cells per side (64x22): lane l of every wave owns window
+// receiver order).  One workgroup per tile (58 x kExTX, the backward rows) of one shot.
+// Its window is the tile plus 3 cells per side (64 x kExCX): lane l of every wave owns window
 // row l, and a thread's cells are columns wave + 8j of its row, so the z neighbours of a
 // cell are the adjacent lanes' values (qw, r and the w mirror use DPP wave shifts;
 // lane_up: lane l gets lane l+1's value, lane_dn: lane l-1's, 0 past the ends, called
@@ -161,8 +167,8 @@ __global__ __launch_bounds__(kThreads) void k_ex_step(
     float *__restrict__ gk, const int *__restrict__ inj_gstart, const InjGroup *__restrict__ inj_groups,
     const RecRef *__restrict__ inj_list, int inj_cap, const float *__restrict__ g) {
     using namespace bk;
-    __shared__ float sP[kNL];
-    __shared__ float sA[kNL];     // qu, then r
+    __shared__ float sP[kExNL];
+    __shared__ float sA[kExNL];     // qu, then r
 
     int s, t;
     work_item(xcd_logical(blockIdx.x, gridDim.x), (int)(gridDim.x / tm.ntiles), tm, s, t);
@@ -177,7 +183,7 @@ __global__ __launch_bounds__(kThreads) void k_ex_step(
     const bool lane_t = lane >= kHalo && z <= tb.z1;           // tile rows
     const int ct_end = tb.x1 - xo;                             // last tile column
     const int m = d.fs - 1 - zo;                               // window row of fs-1
-    constexpr int kJS = (kCX + kWaves - 1) / kWaves;
+    constexpr int kJS = (kExCX + kWaves - 1) / kWaves;
     const float *lin = LIn + (size_t)s * 3 * np;
     float *lout = LOut + (size_t)s * 3 * np;
     float *gks = gk + (size_t)s * np;
@@ -190,7 +196,7 @@ __global__ __launch_bounds__(kThreads) void k_ex_step(
     for (int j = 0; j < kJS; ++j) {
         const int c = wave + kWaves * j;
         lu[j] = lw[j] = qw[j] = 0.f;
-        if (c >= kCX) continue;
+        if (c >= kExCX) continue;
         const int x = xo + c;
         if (zin && x >= 1 && x <= d.nxp) {
             const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
@@ -212,10 +218,10 @@ __global__ __launch_bounds__(kThreads) void k_ex_step(
         kb[j] = make_float2(0.f, 0.f);
         fu[j] = fw[j] = fp[j] = 1.0f;
         dv[j] = gkv[j] = 0.f;
-        if (c >= kCX || !zin || x < 1 || x > d.nxp) continue;
+        if (c >= kExCX || !zin || x < 1 || x > d.nxp) continue;
         const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
         kb[j] = KB[k];
-        if (c >= 2 && c <= kCX - 2 && lane_r) fp[j] = CP[k].y;
+        if (c >= 2 && c <= kExCX - 2 && lane_r) fp[j] = CP[k].y;
         if (c >= kHalo && c <= ct_end && lane_t) {
             const float4 cv = CV[k];
             fu[j] = cv.y;
@@ -250,7 +256,7 @@ __global__ __launch_bounds__(kThreads) void k_ex_step(
         for (int j = 0; j < kJS; ++j) {
             const int c = wave + kWaves * j;
             const int x = xo + c;
-            if (c >= kCX || x < 1 || x > d.nxp) continue;
+            if (c >= kExCX || x < 1 || x > d.nxp) continue;
             const float wm = lane_dn(lw[j]);            // lane m+1 gets lane m's lw
             if (lane == m + 1) lw[j] = lw[j] + wm;
             if (lane == m) lw[j] = 0.0f;
@@ -260,7 +266,7 @@ __global__ __launch_bounds__(kThreads) void k_ex_step(
 #pragma unroll
     for (int j = 0; j < kJS; ++j) {
         const int c = wave + kWaves * j;
-        if (c >= kCX) continue;
+        if (c >= kExCX) continue;
         const int x = xo + c;
         const int l = LW(c, lane);
         const bool g_in = zin && x >= 1 && x <= d.nxp;
@@ -278,7 +284,7 @@ __global__ __launch_bounds__(kThreads) void k_ex_step(
     for (int j = 0; j < kJS; ++j) {
         const int c = wave + kWaves * j;
         const int x = xo + c;
-        if (c < 2 || c > kCX - 2 || x < 1 || x > d.nxp) continue;
+        if (c < 2 || c > kExCX - 2 || x < 1 || x > d.nxp) continue;
         // qw of the rows above and below: the neighbouring lanes' values
         const float qd1 = lane_dn(qw[j]), qd2 = lane_dn(qd1), qu1 = lane_up(qw[j]);
         if (!lane_r || !zin) continue;
@@ -292,7 +298,7 @@ __global__ __launch_bounds__(kThreads) void k_ex_step(
         for (int j = 0; j < kJS; ++j) {
             const int c = wave + kWaves * j;
             const int x = xo + c;
-            if (c < 2 || c > kCX - 2 || x < 1 || x > d.nxp) continue;
+            if (c < 2 || c > kExCX - 2 || x < 1 || x > d.nxp) continue;
             const float pm = sP[LW(c, m)];
             if (m + 2 < kZL && lane == m + 2) sP[LW(c, m + 2)] = sP[LW(c, m + 2)] - pm;
             if (lane == m) sP[LW(c, m)] = 0.0f;
@@ -305,12 +311,12 @@ __global__ __launch_bounds__(kThreads) void k_ex_step(
     for (int j = 0; j < kJS; ++j) {
         const int c = wave + kWaves * j;
         lp[j] = rr[j] = 0.f;
-        if (c >= kCX) continue;
+        if (c >= kExCX) continue;
         const int x = xo + c;
         const int l = LW(c, lane);
         const float p = sP[l];
         lp[j] = p;
-        if (c < 2 || c > kCX - 2 || !lane_r) continue;
+        if (c < 2 || c > kExCX - 2 || !lane_r) continue;
         float r = 0.0f;
         if (zin && x >= 1 && x <= d.nxp && in_rp(d, z, x)) {
             if (c >= kHalo && c <= ct_end && lane_t) gkv[j] = gkv[j] - p * dv[j];

@@ -434,6 +434,7 @@ static TileMap make_tilemap(const Dims &d, int ztop, int th) {
     TileMap tm;
     tm.ztop = ztop;
     tm.th = th;
+    tm.tx = kTX;
     tm.nzt = cdiv(d.nzp - ztop + 1, th);
     tm.nxt = cdiv(d.nxp, kTX);
     tm.ntiles = tm.nzt * tm.nxt;
@@ -442,7 +443,7 @@ static TileMap make_tilemap(const Dims &d, int ztop, int th) {
 }
 
 static int tile_of(const TileMap &tm, int iz, int ix) {
-    int zt = (iz - tm.ztop) / tm.th, xt = (ix - 1) / kTX;
+    int zt = (iz - tm.ztop) / tm.th, xt = (ix - 1) / tm.tx;
     return xt * tm.nzt + zt;
 }
 
@@ -450,10 +451,11 @@ static int tile_of(const TileMap &tm, int iz, int ix) {
 // u/w rows 2..nzp-1, p rows 3..nzp-2; the interior window lies inside), and rows
 // fs-2..fs+1 must share a tile (the adjoint p stencil there reads the mirrored w(fs-1);
 // the p mirror reads row fs+1 after injection), so no tile starts at fs-1..fs+1.  The
-// fused exact-adjoint step uses the same tiles and needs, in every tile's window (tile
-// +- 3 rows), the rows its transposed mirrors read: no tile starts at fs+2 or fs+3
-// either.  Pick the offset with the fewest tiles.
-static TileMap backward_tilemap(const Dims &d) {
+// fused exact-adjoint step uses tiles of the same rows and needs, in every tile's window
+// (tile +- 3 rows), the rows its transposed mirrors read: no tile starts at fs+2 or fs+3
+// either.  Pick the offset with the fewest tiles.  tx: tile width (the fused exact
+// step takes wider tiles than the backward step kernel).
+static TileMap backward_tilemap(const Dims &d, int tx = kTX) {
     const int th = bk::kTZ, fs = d.fs;
     TileMap best{};
     best.ntiles = 1 << 30;
@@ -466,8 +468,9 @@ static TileMap backward_tilemap(const Dims &d) {
         TileMap tm;
         tm.ztop = ztop;
         tm.th = th;
+        tm.tx = tx;
         tm.nzt = cdiv(d.nzp - 1 - ztop + 1, th);           // cover rows up to nzp-1
-        tm.nxt = cdiv(d.nxp, kTX);
+        tm.nxt = cdiv(d.nxp, tx);
         tm.ntiles = tm.nzt * tm.nxt;
         tm.cg = std::min(kColGroup, tm.nxt);
         if (tm.ntiles < best.ntiles) best = tm;
@@ -488,6 +491,7 @@ static TileMap backward2_tilemap(const Dims &d) {
         TileMap tm;
         tm.ztop = 2 - shift;
         tm.th = th;
+        tm.tx = kTX;
         tm.nzt = cdiv(d.nzp - 1 - tm.ztop + 1, th);
         tm.nxt = cdiv(d.nxp, kTX);
         tm.ntiles = tm.nzt * tm.nxt;
@@ -665,7 +669,8 @@ static int batched_init(swit_plan *pl) {
         std::vector<RecRef> el;
         std::vector<InjGroup> eg;
         bs.exinj_cap = 4 * ngmax;
-        TRY(build_injection(pl, pl->rec_cells_fwd, bs.bwd, 3, bs.exinj_cap, es, eg, el));
+        bs.extm = backward_tilemap(d, ex::kExTX);
+        TRY(build_injection(pl, pl->rec_cells_fwd, bs.extm, 3, bs.exinj_cap, es, eg, el));
         TRY(pl->alloc(&bs.exinj_start, es.size()));
         TRY(pl->alloc(&bs.exinj_groups, eg.size()));
         TRY(pl->alloc(&bs.exinj_list, el.size()));
@@ -1431,7 +1436,7 @@ static int exact_adjoint_steps(swit_plan *pl, int a, int b, DSave ds, int &k, hi
     if (bs.exact_fused) {
         float *lam[2] = {bs.lam, bs.lam2};
         for (int it = b; it >= a + 1; --it, k ^= 1) {
-            hipLaunchKernelGGL(ex::k_ex_step, dim3(ns * bs.bwd.ntiles), dim3(kThreads), 0, st, d, it, bs.bwd, bs.KB,
+            hipLaunchKernelGGL(ex::k_ex_step, dim3(ns * bs.extm.ntiles), dim3(kThreads), 0, st, d, it, bs.extm, bs.KB,
                                bs.CP, bs.CV, ds, lam[k], lam[k ^ 1], bs.gk, bs.exinj_start, bs.exinj_groups,
                                bs.exinj_list, bs.exinj_cap, bs.gseis);
             LAUNCH_CHECK();
