@@ -52,14 +52,15 @@ constexpr int kQPT = (kNQ + kThreads - 1) / kThreads;   // region cells per thre
 __device__ __forceinline__ int L(int z, int x, int zo, int xo) { return (x - xo) * kRZ + (z - zo); }
 }  // namespace fk
 
-// Backward step kernel tile: kTZ x kTX outputs.  Its LDS window is one wave tall
+// Backward step kernel tile: kTZ x kBTX outputs.  Its LDS window is one wave tall
 // (kZL = 64 rows: the outputs plus a 3-row halo above and below) and kCX columns wide;
 // lane l of every wave owns window row l, so each row predicate is evaluated once per
 // thread and every column loop is uniform across the wave.
 namespace bk {
 constexpr int kZL = 64;
 constexpr int kTZ = kZL - 2 * kHalo;   // 58 output rows
-constexpr int kCX = kTX + 2 * kHalo;   // 22 window columns
+constexpr int kBTX = 24;               // output columns
+constexpr int kCX = kBTX + 2 * kHalo;  // 30 window columns
 constexpr int kNL = kZL * kCX;         // floats per LDS array
 // LDS index of window column c, row (lane) l.
 __device__ __forceinline__ int LW(int c, int l) { return c * kZL + l; }
@@ -512,7 +513,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
     const bool lane_t = lane >= kHalo && z <= tb.z1;           // tile rows
     // column sets: staged c in [0, kCX); velocities c in [1, cv_end]; tile c in [3, ct_end]
     constexpr int kJS = (kCX + kWaves - 1) / kWaves;
-    constexpr int kJV = (kTX + 3 + kWaves - 1) / kWaves;
+    constexpr int kJV = (kBTX + 3 + kWaves - 1) / kWaves;
     const int cv_end = tb.x1 + 1 - xo;
     const int ct_end = tb.x1 - xo;
 

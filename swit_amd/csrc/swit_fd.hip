@@ -574,7 +574,7 @@ static int batched_init(swit_plan *pl) {
     const Dims &d = pl->d;
     const int ns = pl->ns, ngmax = d.ngmax, fs = d.fs;
     bs.fwd = make_tilemap(d, fs - 1, fk::kTZ);
-    bs.bwd = backward_tilemap(d);
+    bs.bwd = backward_tilemap(d, bk::kBTX);
     bs.bwd2 = backward2_tilemap(d);
     // receivers: recording cells by forward tile
     std::vector<int> rs((size_t)ns * (bs.fwd.ntiles + 1));
