@@ -29,22 +29,23 @@
 
 namespace swit {
 
-constexpr int kTX = 16;          // output columns of a tile (both step kernels)
+constexpr int kTX = 16;          // output columns of a two-step backward tile
 constexpr int kHalo = 3;
 constexpr int kThreads = 512;
 constexpr int kWaves = kThreads / 64;
 constexpr int kNumXcd = 8;
 
-// Forward step kernel tile: kTZ x kTX outputs, LDS window (kTZ+6) x (kTX+6).
+// Forward step kernel tile: kTZ x kFTX outputs, LDS window (kTZ+6) x (kFTX+6).
 namespace fk {
 constexpr int kTZ = 64;
+constexpr int kFTX = 16;
 constexpr int kRZ = kTZ + 2 * kHalo;
-constexpr int kRX = kTX + 2 * kHalo;
+constexpr int kRX = kFTX + 2 * kHalo;
 constexpr int kNR = kRZ * kRX;
-constexpr int kCellsPerThread = kTZ * kTX / kThreads;
-// region of new p the velocity stencils read: (kTZ+3) x (kTX+3) cells
+constexpr int kCellsPerThread = kTZ * kFTX / kThreads;
+// region of new p the velocity stencils read: (kTZ+3) x (kFTX+3) cells
 constexpr int kQZ = kTZ + 3;
-constexpr int kQX = kTX + 3;
+constexpr int kQX = kFTX + 3;
 constexpr int kNQ = kQZ * kQX;
 constexpr int kLPT = (kNR + kThreads - 1) / kThreads;   // staged cells per thread
 constexpr int kQPT = (kNQ + kThreads - 1) / kThreads;   // region cells per thread
@@ -68,7 +69,7 @@ __device__ __forceinline__ int LW(int c, int l) { return c * kZL + l; }
 
 // Tiling of the padded grid: tile (zt, xt) covers rows
 // [max(ztop + zt*th, 1), min(ztop + (zt+1)*th - 1, nzp)] and columns
-// [1 + xt*tx, min((xt+1)*tx, nxp)] (tx = kTX for the step kernels).
+// [1 + xt*tx, min((xt+1)*tx, nxp)].
 struct TileMap {
     int ztop;
     int th;         // tile height (rows)
@@ -401,7 +402,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
         __syncthreads();
         // p mirror (fdcore.f90:92-94) on the region columns
         if (d.free_surface == 1 && fs - 1 >= tb.z0 - 1 && fs + 1 <= tb.z0 + kTZ + 1) {
-            for (int x = tb.x0 - 1 + tid; x <= tb.x0 + kTX + 1; x += kThreads)
+            for (int x = tb.x0 - 1 + tid; x <= tb.x0 + kFTX + 1; x += kThreads)
                 if (x >= 1 && x <= d.nxp) sQ[L(fs - 1, x, zo, xo)] = -sQ[L(fs + 1, x, zo, xo)];
         }
         __syncthreads();

@@ -434,9 +434,9 @@ static TileMap make_tilemap(const Dims &d, int ztop, int th) {
     TileMap tm;
     tm.ztop = ztop;
     tm.th = th;
-    tm.tx = kTX;
+    tm.tx = fk::kFTX;
     tm.nzt = cdiv(d.nzp - ztop + 1, th);
-    tm.nxt = cdiv(d.nxp, kTX);
+    tm.nxt = cdiv(d.nxp, fk::kFTX);
     tm.ntiles = tm.nzt * tm.nxt;
     tm.cg = std::min(kColGroup, tm.nxt);
     return tm;
