@@ -210,6 +210,9 @@ def main():
                          "group with the global-correlation misfit on the device; lbfgs: --iters "
                          "L-BFGS iterations of the reference inversion loop through the drop-in solver "
                          "on the 801x267x6000, 81-shot model with 16-bit strips (--strips, default f16)")
+    ap.add_argument("--bwd-steps", type=int, default=0, choices=[0, 1, 2],
+                    help="reverse steps per backward launch (1 or 2) after autotuning; 0: the autotuner's "
+                         "choice")
     ap.add_argument("--resident", action="store_true",
                     help="with --workload lbfgs: run the device-resident loop (swit_amd.inversion_device) "
                          "instead of the file-based one")
@@ -258,6 +261,8 @@ def main():
     plan = Plan(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, w.src_xz, w.rec_xz, path=args.path, strips=args.strips)
     plan.set_model(vp_true, rho_true)
     tuned = plan.autotune(steps=50) if args.path == "batched" and not args.no_autotune else None
+    if args.bwd_steps and args.path == "batched":
+        plan.set_option("bwd_steps", args.bwd_steps)
     obs = plan.forward(src, save_strips=False).clone()
     torch.cuda.synchronize()
 
@@ -362,7 +367,7 @@ def main():
         "data": "synthetic (seeded 12-layer dipping model with two faults, Ricker 5 Hz)",
         "config": {"workload": f"{w.nx}x{w.nz} grid, npml {w.npml}, nt {w.nt}, dt {w.dt}, {w.ns} shots, "
                                f"{w.ng} receivers, L2 gradient", "path": args.path, "strips": args.strips,
-                   "shots_per_step": w.ns, "autotuned": tuned},
+                   "shots_per_step": w.ns, "autotuned": tuned, "bwd_steps": plan.bwd_steps},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain_baseline": plain,
@@ -464,6 +469,8 @@ def bench_uneven(args):
         p.set_model(torch.from_numpy(w.vp_true).to(dev), torch.from_numpy(w.rho(w.vp_true)).to(dev))
         if not args.no_autotune:
             p.autotune(steps=30)
+        if args.bwd_steps:
+            p.set_option("bwd_steps", args.bwd_steps)
         plans.append(p)
     group = PlanGroup(plans)
     obs = [o.clone() for o in group.forward(srcs, save_strips=False)]

@@ -77,6 +77,10 @@ class DeviceProblem:
             obs = self.plan.forward(self.src, save_strips=0).clone()
         self.obs = self._process(torch.as_tensor(np.asarray(obs) if not torch.is_tensor(obs) else obs,
                                                  dtype=torch.float32).to(self.device).contiguous().clone())
+        # pick the step kernels' launch configuration once, in the starting model (the
+        # results do not depend on it)
+        self.plan.set_model(torch.from_numpy(inp["vp"]).to(self.device), self.rho)
+        self.tuned = self.plan.autotune(steps=20)
 
     def _process(self, gathers):
         if not self.identity:

@@ -40,6 +40,9 @@ STRIPS = "f32"
 # grad_precond of adjoint(): 'device' (GPU kernels, equal to the numpy steps to FP64
 # rounding) or 'host' (the numpy restatement of toolbox/postprocess.py).
 PRECOND = "device"
+# Time the batched step kernels' launch configurations once per synthetic-data plan,
+# at its first model, and keep the fastest (the results do not depend on the choice).
+AUTOTUNE = True
 
 _JOBS = {}
 
@@ -65,13 +68,17 @@ class _Job:
                          free_surface=inputs["free_surface"], path=PATH, strips=STRIPS)
         self.fwd_src = None          # source traces of the last strip-saving forward
         self.model = None            # (vp, rho) the plan currently holds
+        self.tuned = None            # the autotuner's choice, once made
 
-    def set_model(self, vp, rho):
+    def set_model(self, vp, rho, tune=False):
         import torch
         if self.model is not None and np.array_equal(self.model[0], vp) and np.array_equal(self.model[1], rho):
             return
         self.plan.set_model(torch.from_numpy(vp), torch.from_numpy(rho))
         self.model = (vp.copy(), rho.copy())
+        if tune and self.tuned is None and PATH == "batched":
+            # before any strip-saving forward: tuning runs over the plan's buffers
+            self.tuned = self.plan.autotune(steps=20)
 
 
 def solver_inputs(simu):
@@ -144,7 +151,7 @@ def forward(simu, simu_type='obs', savesnap=0):
             # the synthetic plan keeps the boundary strips of a snapshot-saving run for
             # adjoint(); a run without snapshots drops them, as cleandata drops the files
             job = _job(simu, inp)
-            job.set_model(inp["vp"], inp["rho"])
+            job.set_model(inp["vp"], inp["rho"], tune=AUTOTUNE)
             seis = job.plan.forward(torch.from_numpy(src32), save_strips=1 if savesnap == 1 else 0).cpu().numpy()
             job.fwd_src = src32 if savesnap == 1 else None
         else:

@@ -206,6 +206,10 @@ class _RecordingPlan:
     def set_model(self, vp, rho):
         self.log.append(("model", self.id, float(vp.double().sum())))
 
+    def autotune(self, steps=20):
+        self.log.append(("tune", self.id))
+        return {}
+
     def forward(self, src, save_strips):
         self.log.append(("forward", self.id, int(save_strips)))
         return self.torch.zeros(self.ns, self.ng, self.nt)
@@ -235,9 +239,10 @@ def test_dropin_plan_bookkeeping(tmp_path, monkeypatch):
     solver.forward(simu, 'syn', savesnap=1)
     solver.forward(simu_true, 'obs', savesnap=0)
     # the observed run used a plan of its own and closed it; the synthetic plan (0)
-    # saw one model and one strip-saving forward
+    # saw one model, the autotuner (once, before any strip-saving forward) and one
+    # strip-saving forward
     assert [e for e in log if e[1] == 0] == [("new", 0), ("model", 0, float(np.float32(w.vp_init).astype(np.float64).sum())),
-                                             ("forward", 0, 1)]
+                                             ("tune", 0), ("forward", 0, 1)]
     assert [e[0] for e in log if e[1] == 1] == ["new", "model", "forward", "close"]
     preprocess.process_workflow(simu, optim, 'obs')
     preprocess.process_workflow(simu, optim, 'syn')
