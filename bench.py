@@ -321,7 +321,8 @@ def main():
     if args.path == "batched":
         syn_now = plan.forward(src, save_strips=False)
         misfits = misfit_leg(obs, syn_now, w.dt)
-        offsets = np.round(w.rec_xz[:, :, 0]) - np.round(w.src_xz[:, None, 0])
+        from swit_amd.su import nint
+        offsets = (nint(w.rec_xz[:, :, 0]) - nint(w.src_xz[:, None, 0])).astype(np.float64)
         processing = process_leg(syn_now, w.dt, offsets)
 
     autograd = None

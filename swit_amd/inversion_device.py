@@ -20,7 +20,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import inversion as _host
-from . import preprocess
+from . import preprocess, su
 
 
 def _torch():
@@ -62,9 +62,9 @@ class DeviceProblem:
         src = np.array(np.broadcast_to(np.asarray(src, dtype=np.float32), (self.ns, self.nt)))
         self.src = torch.from_numpy(src).to(self.device)
         self.rho = torch.from_numpy(inp["rho"]).to(self.device)
-        # SU header offsets: nint(gx) - nint(sx) (io.f90:687-774)
-        self.offsets = (np.rint(inp["rec_xz"][:, :, 0].astype(np.float64))
-                        - np.rint(inp["src_xz"][:, None, 0].astype(np.float64)))
+        # the SU header's offsets, nint(gx) - nint(sx) (io.f90:687-774), as the file-based
+        # loop's processing reads them
+        self.offsets = (su.nint(inp["rec_xz"][:, :, 0]) - su.nint(inp["src_xz"][:, None, 0])).astype(np.float64)
         self.identity = preprocess._identity(optim)
         self.forwards = 0
         self.adjoints = 0
