@@ -120,9 +120,10 @@ def test_inversion_loop_reduces_misfit(tmp_path, scheme):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind,filt,norm", [("Waveform", "None", "None"),
-                                            ("Globalcorrelation", "Bandpass", ["L2-Trace"])])
-def test_device_loop_matches_file_loop(tmp_path, kind, filt, norm):
+@pytest.mark.parametrize("kind,filt,norm,mute", [("Waveform", "None", "None", False),
+                                                 ("Globalcorrelation", "Bandpass", ["L2-Trace"], False),
+                                                 ("Envelope", "Bandpass", ["Max-Trace"], True)])
+def test_device_loop_matches_file_loop(tmp_path, kind, filt, norm, mute):
     """The device-resident loop (swit_amd.inversion_device: device processing, misfit,
     adjoint source, preconditioning and L-BFGS vectors, no SU files) against the
     file-based loop on the drop-in solver over three L-BFGS iterations: the same
@@ -134,8 +135,9 @@ def test_device_loop_matches_file_loop(tmp_path, kind, filt, norm):
     w.rec_xz = w.rec_xz[:, :w.nx]
 
     def optim():
+        # with `mute`: traces closer than 45 m to the source are zeroed
         return base.optimize(kind, 'LBFGS', 3, 0.05, 3500.0, 1400.0, 'Marine', 3, 2, filt, 3.0, 25.0,
-                             False, 0.0, norm, False, False, 0.0, 1e9)
+                             False, 0.0, norm, mute, False, 45.0, 1e9)
     init = {'vp': w.vp_init.astype(np.float64), 'rho': workloads.gardner(w.vp_init).astype(np.float64)}
     home_a = str(tmp_path / 'a') + "/"
     os.makedirs(home_a)
