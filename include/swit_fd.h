@@ -127,18 +127,20 @@ const char *swit_last_error(void);
 #define SWIT_OPT_EXACT_FUSED 64 /* 1 (default): one fused kernel per exact-adjoint step;
                                    0: the seven separate passes (same results when no
                                    two receivers share a cell)                        */
+#define SWIT_OPT_BWD_TILE_X 128 /* tile width of the one-step backward kernel: 24
+                                   (default) or 16 columns (same results)             */
 int swit_plan_set_option(swit_plan *plan, int option, int value);
 
 /* Autotuner (no counterpart in the reference): times `steps` back-to-back launches of
  * each batched step kernel (best of three bursts) on this plan's grid for every candidate forward
  * shots-per-workgroup (1..8) x column-group order (forward with strip saving) and
- * every backward kernel (one or two steps per launch) x column-group order, and keeps
- * the fastest per time step.  Call between jobs with the model set; it uses the
- * wavefield and strip buffers as scratch, so saved strips (and an exact-adjoint
- * history) are invalidated.
- * choice[4] = {forward shots per workgroup, forward column group, backward column
- * group, backward steps per launch}; us[2] = the chosen forward / backward step-kernel
- * time per time step in microseconds. */
+ * every backward kernel (two steps per launch, or one step per launch at each tile
+ * width) x column-group order, and keeps the fastest per time step.  Call between jobs
+ * with the model set; it uses the wavefield and strip buffers as scratch, so saved
+ * strips (and an exact-adjoint history) are invalidated.
+ * choice[5] = {forward shots per workgroup, forward column group, backward column
+ * group, backward steps per launch, one-step backward tile width}; us[2] = the chosen
+ * forward / backward step-kernel time per time step in microseconds. */
 int swit_plan_autotune(swit_plan *plan, int steps, int *choice, double *us, hipStream_t stream);
 
 /* Accumulated duration and time-step count of the step kernels timed while

@@ -61,6 +61,7 @@ SWIT_OPT_FWD_COLGROUP = 8
 SWIT_OPT_BWD_COLGROUP = 16
 SWIT_OPT_BWD_STEPS = 32
 SWIT_OPT_EXACT_FUSED = 64
+SWIT_OPT_BWD_TILE_X = 128
 
 
 class SwitPrecond(ctypes.Structure):

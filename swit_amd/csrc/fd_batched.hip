@@ -60,7 +60,8 @@ __device__ __forceinline__ int L(int z, int x, int zo, int xo) { return (x - xo)
 namespace bk {
 constexpr int kZL = 64;
 constexpr int kTZ = kZL - 2 * kHalo;   // 58 output rows
-constexpr int kBTX = 24;               // output columns
+constexpr int kBTX = 24;               // output columns (the default width)
+constexpr int kBTXs[2] = {16, 24};     // widths the kernel is built for (autotuner candidates)
 constexpr int kCX = kBTX + 2 * kHalo;  // 30 window columns
 constexpr int kNL = kZL * kCX;         // floats per LDS array
 // LDS index of window column c, row (lane) l.
@@ -467,8 +468,8 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
 // tiles whose updated cells reach a damped cell; elsewhere they are exactly 1.
 // RIn/ROut: [ns][3][nx][nz]; BIn/BOut: [ns][3][nxp][ld].
 // acc: [ns][3][nx][nz] FP32 per-shot accumulators (kernel, forward illumination,
-// backward illumination), updated in the reference's order.
-template <int SM>
+// backward illumination), updated in the reference's order.  TX: tile width (columns).
+template <int SM, int TX>
 __global__ __launch_bounds__(kThreads) void k_bwd_fused(
     Dims d, int it, TileMap tm, const float2 *__restrict__ KB, const float2 *__restrict__ CP,
     const float4 *__restrict__ CV,
@@ -478,6 +479,8 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
     float *__restrict__ BOut, const int *__restrict__ inj_gstart, const InjGroup *__restrict__ inj_groups,
     const RecRef *__restrict__ inj_list, const float *__restrict__ sadj_t, float *__restrict__ acc) {
     using namespace bk;
+    constexpr int kCX = TX + 2 * kHalo;   // window columns
+    constexpr int kNL = kZL * kCX;        // floats per LDS array
     __shared__ float rP[kNL];
     __shared__ float rU[kNL];
     __shared__ float rW[kNL];
@@ -514,7 +517,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
     const bool lane_t = lane >= kHalo && z <= tb.z1;           // tile rows
     // column sets: staged c in [0, kCX); velocities c in [1, cv_end]; tile c in [3, ct_end]
     constexpr int kJS = (kCX + kWaves - 1) / kWaves;
-    constexpr int kJV = (kBTX + 3 + kWaves - 1) / kWaves;
+    constexpr int kJV = (TX + 3 + kWaves - 1) / kWaves;
     const int cv_end = tb.x1 + 1 - xo;
     const int ct_end = tb.x1 - xo;
 
@@ -1083,6 +1086,13 @@ struct BatchedState {
     int *inj_start = nullptr;    // [ns][bwd.ntiles+1] group ranges per tile
     InjGroup *inj_groups = nullptr;  // [ns][ngmax]
     RecRef *inj_list = nullptr;  // [ns][ngmax]
+    // one-step backward tilings of every width in bk::kBTXs with their injection groups;
+    // bwd and inj_* above are those of width index bwd_wi
+    TileMap bwdw[2]{};
+    int *inj_start_w[2] = {nullptr, nullptr};
+    InjGroup *inj_groups_w[2] = {nullptr, nullptr};
+    RecRef *inj_list_w[2] = {nullptr, nullptr};
+    int bwd_wi = 1;
     // two-step backward kernel: its tiling, the injection groups of its tiles (step 2)
     // and of its tiles grown by 3 cells per side (step 1, per-shot capacity injx_cap)
     TileMap bwd2{};

@@ -574,7 +574,9 @@ static int batched_init(swit_plan *pl) {
     const Dims &d = pl->d;
     const int ns = pl->ns, ngmax = d.ngmax, fs = d.fs;
     bs.fwd = make_tilemap(d, fs - 1, fk::kTZ);
-    bs.bwd = backward_tilemap(d, bk::kBTX);
+    for (int wi = 0; wi < 2; ++wi) bs.bwdw[wi] = backward_tilemap(d, bk::kBTXs[wi]);
+    bs.bwd_wi = bk::kBTXs[0] == bk::kBTX ? 0 : 1;
+    bs.bwd = bs.bwdw[bs.bwd_wi];
     bs.bwd2 = backward2_tilemap(d);
     // receivers: recording cells by forward tile
     std::vector<int> rs((size_t)ns * (bs.fwd.ntiles + 1));
@@ -598,11 +600,25 @@ static int batched_init(swit_plan *pl) {
     }
     // injection cells: tiles of the one-step kernel, tiles of the two-step kernel, and
     // those grown by 3 cells per side (a receiver lies in at most 2 x 2 of them)
-    std::vector<int> is, is2, isx;
-    std::vector<RecRef> il, il2, ilx;
-    std::vector<InjGroup> ig, ig2, igx;
+    std::vector<int> is2, isx;
+    std::vector<RecRef> il2, ilx;
+    std::vector<InjGroup> ig2, igx;
     bs.injx_cap = 4 * ngmax;
-    TRY(build_injection(pl, pl->rec_cells_inj, bs.bwd, 0, ngmax, is, ig, il));
+    for (int wi = 0; wi < 2; ++wi) {
+        std::vector<int> is;
+        std::vector<RecRef> il;
+        std::vector<InjGroup> ig;
+        TRY(build_injection(pl, pl->rec_cells_inj, bs.bwdw[wi], 0, ngmax, is, ig, il));
+        TRY(pl->alloc(&bs.inj_start_w[wi], is.size()));
+        TRY(pl->alloc(&bs.inj_list_w[wi], il.size()));
+        TRY(pl->alloc(&bs.inj_groups_w[wi], ig.size()));
+        HIP_TRY(hipMemcpy(bs.inj_start_w[wi], is.data(), sizeof(int) * is.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(bs.inj_list_w[wi], il.data(), sizeof(RecRef) * il.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(bs.inj_groups_w[wi], ig.data(), sizeof(InjGroup) * ig.size(), hipMemcpyHostToDevice));
+    }
+    bs.inj_start = bs.inj_start_w[bs.bwd_wi];
+    bs.inj_list = bs.inj_list_w[bs.bwd_wi];
+    bs.inj_groups = bs.inj_groups_w[bs.bwd_wi];
     TRY(build_injection(pl, pl->rec_cells_inj, bs.bwd2, 0, ngmax, is2, ig2, il2));
     TRY(build_injection(pl, pl->rec_cells_inj, bs.bwd2, 3, bs.injx_cap, isx, igx, ilx));
     // strip slots by forward tile
@@ -627,9 +643,6 @@ static int batched_init(swit_plan *pl) {
     const size_t np = (size_t)d.nxp * d.ld, ni = (size_t)d.nx * d.nz;
     TRY(pl->alloc(&bs.rec_start, rs.size()));
     TRY(pl->alloc(&bs.rec_list, rl.size()));
-    TRY(pl->alloc(&bs.inj_start, is.size()));
-    TRY(pl->alloc(&bs.inj_list, il.size()));
-    TRY(pl->alloc(&bs.inj_groups, ig.size()));
     TRY(pl->alloc(&bs.inj2_start, is2.size()));
     TRY(pl->alloc(&bs.inj2_list, il2.size()));
     TRY(pl->alloc(&bs.inj2_groups, ig2.size()));
@@ -687,9 +700,6 @@ static int batched_init(swit_plan *pl) {
     }
     HIP_TRY(hipMemcpy(bs.rec_start, rs.data(), sizeof(int) * rs.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(bs.rec_list, rl.data(), sizeof(RecRef) * rl.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(bs.inj_start, is.data(), sizeof(int) * is.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(bs.inj_list, il.data(), sizeof(RecRef) * il.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(bs.inj_groups, ig.data(), sizeof(InjGroup) * ig.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(bs.inj2_start, is2.data(), sizeof(int) * is2.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(bs.inj2_list, il2.data(), sizeof(RecRef) * il2.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(bs.inj2_groups, ig2.data(), sizeof(InjGroup) * ig2.size(), hipMemcpyHostToDevice));
@@ -815,9 +825,22 @@ static void launch_bwd(swit_plan *pl, int it, const float *ri, float *ro, const 
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
     const void *strips = SM == 0 ? static_cast<const void *>(pl->strips) : static_cast<const void *>(bs.strips16);
-    hipLaunchKernelGGL(k_bwd_fused<SM>, dim3(pl->ns * bs.bwd.ntiles), dim3(kThreads), 0, st, d, it, bs.bwd,
-                       bs.KB, bs.CP, bs.CV, pl->geo, bs.src, strips, bs.smax, ri, ro, bi, bo, bs.inj_start,
-                       bs.inj_groups, bs.inj_list, pl->sadj_t, bs.acc);
+    auto kern = bs.bwd.tx == bk::kBTXs[0] ? k_bwd_fused<SM, bk::kBTXs[0]> : k_bwd_fused<SM, bk::kBTXs[1]>;
+    hipLaunchKernelGGL(kern, dim3(pl->ns * bs.bwd.ntiles), dim3(kThreads), 0, st, d, it, bs.bwd, bs.KB, bs.CP,
+                       bs.CV, pl->geo, bs.src, strips, bs.smax, ri, ro, bi, bo, bs.inj_start, bs.inj_groups,
+                       bs.inj_list, pl->sadj_t, bs.acc);
+}
+
+// Make width index wi (of bk::kBTXs) the one-step backward kernel's tiling; the column
+// group is kept where the new tiling allows it.
+static void select_bwd_width(BatchedState &bs, int wi) {
+    const int cg = bs.bwd.cg;
+    bs.bwd_wi = wi;
+    bs.bwd = bs.bwdw[wi];
+    bs.bwd.cg = std::min(cg, bs.bwd.nxt);
+    bs.inj_start = bs.inj_start_w[wi];
+    bs.inj_list = bs.inj_list_w[wi];
+    bs.inj_groups = bs.inj_groups_w[wi];
 }
 
 template <int SM>
@@ -1062,25 +1085,33 @@ extern "C" int swit_plan_autotune(swit_plan *pl, int steps, int *choice, double 
             if (t < best_f) { best_f = t; bg = G; bcf = cg; }
         }
     }
-    int bsteps = bs.bwd_steps;
+    int bsteps = bs.bwd_steps, bwi = bs.bwd_wi;
+    const int wi0 = bs.bwd_wi;
     for (int sp = 1; sp <= 2; ++sp) {
         if (sp == 2 && pl->d.nt < 3) continue;
         bs.bwd_steps = sp;
-        for (int cg : kCols) {
-            if (cg > bs.bwd.nxt) continue;
-            bs.bwd.cg = bs.bwd2.cg = cg;
-            double t = 1e30;
-            for (int r = 0; r < kRepeats; ++r) {
-                double tr;
-                TRY(time_steps(pl, 1, steps, stream, &tr));
-                t = std::min(t, tr);
+        // the one-step kernel's tile width is a candidate too; the two-step kernel keeps
+        // its own tiling
+        for (int wi = 0; wi < 2; ++wi) {
+            if (sp == 2 && wi != wi0) continue;
+            select_bwd_width(bs, wi);
+            for (int cg : kCols) {
+                if (cg > bs.bwd.nxt || (sp == 2 && cg > bs.bwd2.nxt)) continue;
+                bs.bwd.cg = bs.bwd2.cg = cg;
+                double t = 1e30;
+                for (int r = 0; r < kRepeats; ++r) {
+                    double tr;
+                    TRY(time_steps(pl, 1, steps, stream, &tr));
+                    t = std::min(t, tr);
+                }
+                if (t < best_b) { best_b = t; bcb = cg; bsteps = sp; bwi = wi; }
             }
-            if (t < best_b) { best_b = t; bcb = cg; bsteps = sp; }
         }
     }
     bs.Gf = bg;
     bs.ngroups_f = cdiv(pl->ns, bg);
     bs.fwd.cg = bcf;
+    select_bwd_width(bs, bwi);
     bs.bwd.cg = bs.bwd2.cg = bcb;
     bs.bwd_steps = bsteps;
     drop_graphs(bs);
@@ -1088,6 +1119,7 @@ extern "C" int swit_plan_autotune(swit_plan *pl, int steps, int *choice, double 
     choice[1] = bcf;
     choice[2] = bcb;
     choice[3] = bsteps;
+    choice[4] = bs.bwd.tx;
     us[0] = best_f;
     us[1] = best_b;
     return SWIT_OK;
@@ -1121,6 +1153,16 @@ extern "C" int swit_plan_set_option(swit_plan *pl, int option, int value) {
         if (value < 1 || value > tm.nxt) return set_err(SWIT_EINVAL, "column group %d outside [1, %d]", value, tm.nxt);
         tm.cg = value;
         if (option == SWIT_OPT_BWD_COLGROUP) bs.bwd2.cg = value;
+        drop_graphs(bs);
+        return SWIT_OK;
+    }
+    case SWIT_OPT_BWD_TILE_X: {
+        int wi = -1;
+        for (int k = 0; k < 2; ++k)
+            if (bk::kBTXs[k] == value) wi = k;
+        if (wi < 0) return set_err(SWIT_EINVAL, "backward tile width %d is not one of %d, %d", value, bk::kBTXs[0],
+                                   bk::kBTXs[1]);
+        select_bwd_width(bs, wi);
         drop_graphs(bs);
         return SWIT_OK;
     }

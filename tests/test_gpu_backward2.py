@@ -1,5 +1,6 @@
 """Two-step backward kernel (two reverse time steps per launch) against the one-step
-kernel and the oracle: identical bits for odd and even step counts, every strip mode,
+kernel at both its tile widths and the oracle: identical bits for odd and even step
+counts, every strip mode,
 free-surface rows at many offsets inside the two-step windows, and receivers on every
 row of columns at and around tile edges (adjoint sources injected in both steps'
 regions, receivers sharing cells added in receiver order)."""
@@ -11,17 +12,19 @@ from oracle import oracle as O
 from swit_amd import workloads
 
 
-def _receivers(w, tile=16):
+def _receivers(w, tiles=(16, 24)):
     """Surface row, plus full-depth lines on interior columns whose padded column lies
-    at or next to a tile edge, plus duplicates of a few cells."""
+    at or next to a tile edge (of each tile width), plus duplicates of a few cells."""
     nxp = w.nx + 2 * w.npml
     cols = []
-    for edge in range(tile, nxp, tile):                  # last padded column of a tile
-        for ix in (edge - 2, edge, edge + 1, edge + 3):
-            jx = ix - w.npml                             # interior column (1-based)
-            if 1 <= jx <= w.nx and jx - 1 not in cols:
-                cols.append(jx - 1)
-    cols = cols[:6]
+    for tile in tiles:
+        found = []
+        for edge in range(tile, nxp, tile):              # last padded column of a tile
+            for ix in (edge - 2, edge, edge + 1, edge + 3):
+                jx = ix - w.npml                         # interior column (1-based)
+                if 1 <= jx <= w.nx and jx - 1 not in cols and jx - 1 not in found:
+                    found.append(jx - 1)
+        cols += found[:4]
     rec = [[x * w.dx, 0.0] for x in range(w.nx)]
     for c in cols:
         rec += [[c * w.dx, z * w.dx] for z in range(1, w.nz - 1)]
@@ -59,15 +62,18 @@ def test_two_step_backward_matches_one_step(nx, nz, npml, nt, ns, strips):
     m = O.Model(w.vp_true, workloads.gardner(w.vp_true), w.dx, w.npml, "f32")
     obs = O.forward(m, w.nt, w.dt, w.src_xz, w.rec_xz, src, save_strips=False)["seis"]
     out = {}
-    for steps in (1, 2):
+    # the one-step kernel at both its tile widths, and the two-step kernel
+    for steps, tx in ((1, 16), (1, 24), (2, 24)):
         for graph in (1, 0):
             p = _plan(w, strips, steps)
+            p.set_option("bwd_tile_x", tx)
             p.set_option("graph", graph)
             syn = p.forward(torch.from_numpy(src), save_strips=True)
             adj = (syn.cpu().numpy() - obs).astype(np.float32)
-            out[steps, graph] = [t.cpu().numpy() for t in p.adjoint(torch.from_numpy(src), torch.from_numpy(adj))]
+            out[steps, tx, graph] = [t.cpu().numpy() for t in p.adjoint(torch.from_numpy(src),
+                                                                        torch.from_numpy(adj))]
             p.close()
-    ref = out[1, 1]
+    ref = out[1, 24, 1]
     for key, g in out.items():
         for x, y in zip(g, ref):
             np.testing.assert_array_equal(x, y, err_msg=str(key))
@@ -98,7 +104,13 @@ def test_two_step_is_the_default_and_profiles_per_step():
         assert bn == w.nt - 1 and bm > 0
         res[steps] = g
     np.testing.assert_array_equal(res[1], res[2])
+    from swit_amd import _native
     from swit_amd.engine import Plan
     p = Plan(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, w.src_xz, w.rec_xz)
     assert p.bwd_steps == 2
+    with pytest.raises(_native.SwitError):
+        p.set_option("bwd_tile_x", 20)
+    p.set_model(torch.from_numpy(w.vp_init), torch.from_numpy(workloads.gardner(w.vp_init)))
+    tuned = p.autotune(steps=4)
+    assert tuned["bwd_tile_x"] in (16, 24) and tuned["bwd_steps"] in (1, 2)
     p.close()
