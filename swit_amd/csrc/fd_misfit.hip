@@ -113,8 +113,9 @@ __global__ __launch_bounds__(kMisfitThreads) void k_sum_fixed(int n, const doubl
 // stage with radix R after ns points of earlier stages is
 //   sum_r x[j + r nt/R] W^(r (jm + q ns) nt/(ns R)),  jm = o mod ns, q = (o/ns) mod R,
 //   j = (o / (ns R)) ns + jm,  W = exp(-+2 pi i / nt),
-// and twiddles come from two LDS tables of about sqrt(nt) entries each:
-// W^e = A[e / s] B[e mod s].  Two real traces go through one complex transform.
+// W^(r step) is formed by repeated products from W^step, which comes from two LDS
+// tables of about sqrt(nt) entries each: W^e = A[e / s] B[e mod s].  Two real traces go
+// through one complex transform.
 constexpr int kFftMaxStages = 24;
 constexpr int kFftMaxN = 9216;
 constexpr int kFftTw = 96;      // ceil(sqrt(kFftMaxN)) and ceil(kFftMaxN / that)
@@ -156,7 +157,7 @@ __device__ void fft_twiddles(FftLds<NMAX> &L, const FftFactors &f) {
 template <int NMAX, bool INV>
 __device__ void fft_lds(FftLds<NMAX> &L, const FftFactors &f) {
     constexpr int T = fft_threads<NMAX>(), E = (NMAX + T - 1) / T;
-    const int n = f.n, s = f.s, nh = n / s, nl = n % s;
+    const int n = f.n, s = f.s;
     int ns = 1;
     __syncthreads();
     for (int st = 0; st < f.nst; ++st) {
@@ -170,23 +171,20 @@ __device__ void fft_lds(FftLds<NMAX> &L, const FftFactors &f) {
                 const int jm = o % ns, q = (o / ns) % R;
                 const int j = (o / span) * ns + jm;
                 const int step = (jm + q * ns) * mult;        // < n
-                const int sh = step / s, sl = step % s;
-                int eh = 0, el = 0;                           // exponent eh s + el, kept < n
+                // w1 = W^step from the tables; W^(r step) by repeated products
+                const double2 a = L.ta[step / s], b = L.tb[step % s];
+                const double w1r = a.x * b.x - a.y * b.y;
+                const double w1i = INV ? -(a.x * b.y + a.y * b.x) : a.x * b.y + a.y * b.x;
+                // fused multiply-adds (explicit: the library is built without
+                // contraction, which the stencils need; here it only removes roundings)
+                double wr = 1.0, wi = 0.0;
                 for (int r = 0; r < R; ++r) {
                     const double2 v = L.x[j + r * nr];
-                    const double2 a = L.ta[eh], b = L.tb[el];
-                    double wr = a.x * b.x - a.y * b.y, wi = a.x * b.y + a.y * b.x;
-                    if (INV) wi = -wi;
-                    re += v.x * wr - v.y * wi;
-                    im += v.x * wi + v.y * wr;
-                    el += sl;
-                    eh += sh;
-                    if (el >= s) { el -= s; ++eh; }
-                    if (eh > nh || (eh == nh && el >= nl)) {
-                        eh -= nh;
-                        el -= nl;
-                        if (el < 0) { el += s; --eh; }
-                    }
+                    re = fma(v.x, wr, fma(-v.y, wi, re));
+                    im = fma(v.x, wi, fma(v.y, wr, im));
+                    const double t = fma(wr, w1r, -wi * w1i);
+                    wi = fma(wr, w1i, wi * w1r);
+                    wr = t;
                 }
             }
             acc[k] = make_double2(re, im);
