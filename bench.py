@@ -204,12 +204,16 @@ def main():
     ap.add_argument("--cpu-shots", type=int, default=0, help="CPU baseline sample size (0: auto)")
     ap.add_argument("--plain-shots", type=int, default=2, help="plain-path comparison sample (0: skip)")
     ap.add_argument("--iters", type=int, default=10, help="L-BFGS iterations of --workload lbfgs")
-    ap.add_argument("--workload", default="c2", choices=["c2", "uneven", "lbfgs"],
+    ap.add_argument("--out", default="gpurun_out/precision_c5.npz",
+                    help="with --workload precision: where the two-shot gradients go")
+    ap.add_argument("--workload", default="c2", choices=["c2", "uneven", "lbfgs", "precision"],
                     help="c2: the 481x141x4000, 49-shot gradient (default); uneven: three grid "
                          "families (501x171x5000, 481x121x4000, 301x101x3000; 64 shots) as one plan "
                          "group with the global-correlation misfit on the device; lbfgs: --iters "
                          "L-BFGS iterations of the reference inversion loop through the drop-in solver "
-                         "on the 801x267x6000, 81-shot model with 16-bit strips (--strips, default f16)")
+                         "on the 801x267x6000, 81-shot model with 16-bit strips (--strips, default f16); "
+                         "precision: two shots of that model with FP32 and 16-bit strips, gradients to --out "
+                         "for tests/precision_c5.py")
     ap.add_argument("--bwd-steps", type=int, default=0, choices=[0, 1, 2],
                     help="reverse steps per backward launch (1 or 2) after autotuning; 0: the autotuner's "
                          "choice")
@@ -242,6 +246,8 @@ def main():
         return bench_uneven(args)
     if args.workload == "lbfgs":
         return bench_lbfgs(args)
+    if args.workload == "precision":
+        return bench_precision(args)
 
     import torch
     from swit_amd import workloads
@@ -448,6 +454,39 @@ def bench_lbfgs(args):
         "observed_data_s": round(t_obs, 2),
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_precision(args):
+    """GPU half of the 801x267x6000 precision sample: two shots (the first and the
+    middle one) of the 81-shot workload, L2 gradient with FP32 and 16-bit strips, the
+    waveform misfit; written to --out (npz) for tests/precision_c5.py, which runs the
+    FP64 oracle on the same two shots in a process of its own (no GPU runtime there)."""
+    import torch
+    from scipy.integrate import cumulative_trapezoid
+    from swit_amd import workloads
+    from swit_amd.engine import Plan, misfit
+    w = workloads.config5()
+    sel = [0, w.ns // 2]
+    dev = torch.device("cuda", 0)
+    src = cumulative_trapezoid(workloads.ricker(w.nt, w.dt, w.f0), initial=0).astype(np.float32)
+    src = torch.from_numpy(np.broadcast_to(src, (len(sel), w.nt)).copy()).to(dev)
+    out = {"sel": np.array(sel)}
+    for st in ("f32", "f16", "bf16"):
+        p = Plan(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, w.src_xz[sel], w.rec_xz[sel], strips=st)
+        p.set_model(torch.from_numpy(w.vp_true).to(dev), torch.from_numpy(w.rho(w.vp_true)).to(dev))
+        obs = p.forward(src, save_strips=False).clone()
+        p.set_model(torch.from_numpy(w.vp_init).to(dev), torch.from_numpy(w.rho(w.vp_init)).to(dev))
+        syn = p.forward(src, save_strips=True)
+        J, adj, _ = misfit("Waveform", obs, syn, w.dt)
+        g, fw, bk = p.adjoint(src, adj)
+        out["grad_" + st] = g.cpu().numpy()
+        out["misfit_" + st] = float(J)
+        torch.cuda.synchronize()
+        p.close()
+        print(f"{st}: misfit {out['misfit_' + st]:.9e}", flush=True)
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    np.savez(args.out, **out)
+    print(json.dumps({"workload": "801x267x6000 precision sample, shots " + str(sel), "out": args.out}))
 
 
 def bench_uneven(args):
