@@ -219,17 +219,12 @@ def LBFGS(inv_scheme):
 
 # -- line search, decisions, state (inversion.py:266-474) ------------------------------------
 def model_update(vp, update, vpmax, vpmin):
-    """inversion.py:465-474: step and clip to [vpmin, vpmax]."""
-    return (vp + update).clamp(min=vpmin, max=vpmax) if vpmin <= vpmax else _clip_inverted(vp + update, vpmax, vpmin)
-
-
-def _clip_inverted(v, vpmax, vpmin):
-    """The reference's two masked assignments when vpmin > vpmax: every value ends at
-    vpmin or vpmax in that order (values above vpmax first become vpmax < vpmin)."""
-    v = v.clone()
-    v[v > vpmax] = vpmax
-    v[v < vpmin] = vpmin
-    return v
+    """inversion.py:465-474: step, then the reference's two masked assignments in its
+    order (values above vpmax become vpmax, then values below vpmin become vpmin)."""
+    torch = _torch()
+    v = vp + update
+    v = torch.where(v > vpmax, torch.full_like(v, vpmax), v)
+    return torch.where(v < vpmin, torch.full_like(v, vpmin), v)
 
 
 def backtrack(problem, optim, inv_scheme):
