@@ -12,9 +12,9 @@ backward step kernel, which does two reverse time steps per launch unless the au
 picks the one-step kernel; its average duration comes from HIP events on the launch
 stream that bracket each time loop of the timed steps) and a CPU baseline (the FP32
 oracle on the host cores, bounded sample).  Side legs, outside the timed steps: the
-plain per-shot path on a sample of shots, the same gradient through torch.autograd
-with the exact adjoint, and the four device misfits and the device shot-gather
-processing on the step's gathers.
+plain per-shot path on a sample of shots, the same gradient on two concurrent lanes
+(engine.Lanes), the same gradient through torch.autograd with the exact adjoint, and
+the four device misfits and the device shot-gather processing on the step's gathers.
 """
 from __future__ import annotations
 
@@ -102,6 +102,33 @@ def autograd_leg(w, src, obs, vp_init, rho_init, args):
             "algorithmic_bytes_per_gradient": int(per_grad), "achieved_gbs": round(gbs, 1),
             "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
             "finite": bool(torch.isfinite(g).all())}
+
+
+def lanes_leg(w, src, obs, vp, rho, g_ref, args, lanes=2):
+    """The same gradient on engine.Lanes: the shots split into `lanes` plans, each
+    running forward, adjoint source and adjoint on a stream of its own, so that one
+    lane's adjoint overlaps another's forward.  Timed as the main steps (warm-up, then
+    `steps` gradients between synchronisations); the gradient is checked against the
+    main step's (FP64 sums in another order)."""
+    import torch
+    from swit_amd.engine import Lanes
+    L = Lanes(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, w.src_xz, w.rec_xz, lanes=lanes, strips=args.strips)
+    L.set_model(torch.from_numpy(w.vp_true).to(vp.device), torch.from_numpy(w.rho(w.vp_true)).to(vp.device))
+    tuned = L.autotune(steps=50) if not args.no_autotune else None
+    L.set_model(vp, rho)
+    for _ in range(args.warmup):
+        L.gradient(src, obs, "Waveform", w.dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        J, g, fw, bk = L.gradient(src, obs, "Waveform", w.dt)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    rel = float((g - g_ref).norm() / g_ref.norm())
+    L.close()
+    return {"lanes": lanes, "value": round(w.ns * args.steps / el, 3), "unit": "shot-gradients/s",
+            "ms_per_step": round(1000 * el / args.steps, 2), "grad_rel_vs_main": rel,
+            "autotuned": [t.get("bwd_steps") for t in tuned] if tuned else None}
 
 
 def misfit_leg(obs, syn, dt, reps=5):
@@ -323,6 +350,7 @@ def main():
                     "time_loops_gbs": round((kern["forward"]["step_bytes"] * f_n + kern["backward"]["step_bytes"]
                                              * b_n) / ((f_ms + b_ms) * 1e-3) / 1e9, 1)}
 
+    lanes = lanes_leg(w, src, obs, vp_init, rho_init, g[0], args) if args.path == "batched" else None
     misfits = processing = None
     if args.path == "batched":
         syn_now = plan.forward(src, save_strips=False)
@@ -379,6 +407,7 @@ def main():
         "cpu_baseline": cpu,
         "plain_baseline": plain,
         "autograd": autograd,
+        "concurrent_lanes": lanes,
         "device_misfits": misfits,
         "device_processing": processing,
     }

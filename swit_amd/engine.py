@@ -263,6 +263,113 @@ class PlanGroup:
             pass
 
 
+class Lanes:
+    """The shots of one job split into `lanes` batched plans (contiguous shot ranges),
+    each running its whole job on a stream of its own.  A gradient on two lanes lets
+    one lane's adjoint overlap the other's forward: at 481x141x4000 with 49 shots it runs
+    in 473 ms against 514 ms for one plan (three or four lanes were slower).  Gradients
+    and illuminations are the lanes' FP64 sums added in lane order, so they equal the
+    one-plan result to FP64 rounding.  Plan keyword arguments (strips, ...) apply to
+    every lane."""
+
+    def __init__(self, nx, nz, dx, dt, nt, npml, src_xz, rec_xz, lanes=2, **plan_kw):
+        torch = _require_gpu()
+        self._torch = torch
+        src_xz = np.asarray(src_xz, dtype=np.float32).reshape(-1, 2)
+        ns = src_xz.shape[0]
+        ng, rec = pack_receivers(rec_xz, ns)
+        lanes = max(1, min(int(lanes), ns))
+        b = np.linspace(0, ns, lanes + 1).round().astype(int)
+        self.ranges = [(int(b[i]), int(b[i + 1])) for i in range(lanes)]
+        self.plans = [Plan(nx, nz, dx, dt, nt, npml, src_xz[a:e], [rec[i, :ng[i]] for i in range(a, e)], **plan_kw)
+                      for a, e in self.ranges]
+        self.streams = [torch.cuda.Stream() for _ in self.ranges]
+        self.ns, self.nx, self.nz, self.nt = ns, int(nx), int(nz), int(nt)
+        self.ngmax = max(p.ngmax for p in self.plans)
+        self.device = self.plans[0].device
+
+    def _run(self, fn):
+        """fn(i, plan, shot range) on lane i's stream, every lane; returns the results
+        after the caller's stream has waited for all lanes.  Device tensors in the
+        results are marked as used by the caller's stream, so their memory is not
+        handed out again on a lane's stream before the caller is done with them."""
+        torch = self._torch
+        cur = torch.cuda.current_stream(self.device)
+        out = []
+        for i, (p, r, st) in enumerate(zip(self.plans, self.ranges, self.streams)):
+            st.wait_stream(cur)
+            with torch.cuda.stream(st):
+                out.append(fn(i, p, r))
+        for st in self.streams:
+            cur.wait_stream(st)
+        for res in out:
+            for t in (res if isinstance(res, (tuple, list)) else (res,)):
+                if torch.is_tensor(t) and t.is_cuda:
+                    t.record_stream(cur)
+        return out
+
+    def set_model(self, vp, rho):
+        torch = self._torch
+        vp = self.plans[0]._dev(vp, torch.float32, (self.nx, self.nz))
+        rho = self.plans[0]._dev(rho, torch.float32, (self.nx, self.nz))
+        self._run(lambda i, p, r: p.set_model(vp, rho))
+
+    def autotune(self, steps=20):
+        """Each lane's plan tuned alone, one after the other (the timings must not share
+        the GPU)."""
+        return [p.autotune(steps) for p in self.plans]
+
+    def forward(self, src, save_strips=True):
+        """src (ns, nt) -> seismograms (ns, ngmax, nt) of every shot."""
+        torch = self._torch
+        src = self.plans[0]._dev(src, torch.float32, (self.ns, self.nt))
+        outs = self._run(lambda i, p, r: p.forward(src[r[0]:r[1]], save_strips=save_strips))
+        seis = torch.zeros((self.ns, self.ngmax, self.nt), dtype=torch.float32, device=self.device)
+        for (a, e), o in zip(self.ranges, outs):
+            seis[a:e, :o.shape[1]] = o
+        return seis
+
+    def gradient(self, src, obs, kind, dt, process=None):
+        """Per lane: forward with strips, process(lane, shot range, gathers) if given (in
+        place), misfit `kind` and adjoint source against obs[range] (engine.misfit), and
+        the adjoint.  Returns (misfit, grad, illum_forw, illum_back), FP64 sums over the
+        lanes in lane order."""
+        torch = self._torch
+        src = self.plans[0]._dev(src, torch.float32, (self.ns, self.nt))
+
+        def lane(i, p, r):
+            syn = p.forward(src[r[0]:r[1]], save_strips=True)
+            if process is not None:
+                process(i, r, syn)
+            J, adj, _ = misfit(kind, obs[r[0]:r[1], :p.ngmax], syn, dt)
+            return (J,) + p.adjoint(src[r[0]:r[1]], adj)
+        outs = self._run(lane)
+        tot = [o for o in outs[0]]
+        for o in outs[1:]:
+            tot = [a + b for a, b in zip(tot, o)]
+        return tuple(tot)
+
+    def misfit(self, src, obs, kind, dt, process=None):
+        """Forward without strips, optional processing, misfit; the lanes' sum."""
+        torch = self._torch
+        src = self.plans[0]._dev(src, torch.float32, (self.ns, self.nt))
+
+        def lane(i, p, r):
+            syn = p.forward(src[r[0]:r[1]], save_strips=False)
+            if process is not None:
+                process(i, r, syn)
+            return misfit(kind, obs[r[0]:r[1], :p.ngmax], syn, dt)[0]
+        outs = self._run(lane)
+        J = outs[0]
+        for x in outs[1:]:
+            J = J + x
+        return J
+
+    def close(self):
+        for p in self.plans:
+            p.close()
+
+
 _MISFIT_KIND = {"waveform": _native.SWIT_MISFIT_WAVEFORM,
                 "globalcorrelation": _native.SWIT_MISFIT_GLOBAL_CORRELATION,
                 "envelope": _native.SWIT_MISFIT_ENVELOPE,

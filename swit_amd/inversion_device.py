@@ -35,19 +35,20 @@ def _dot(a, b):
 
 
 class DeviceProblem:
-    """The FWI problem of a SWIT simulation on one GPU: a batched plan of simu's grid and
-    acquisition, the integrated source (toolbox/solver.py:41, cumulative trapezoid), the
-    processed observed gathers and optim's processing, misfit and preconditioning.
+    """The FWI problem of a SWIT simulation on one GPU: batched plans of simu's grid and
+    acquisition (`lanes` shot ranges on streams of their own, engine.Lanes), the
+    integrated source (toolbox/solver.py:41, cumulative trapezoid), the processed
+    observed gathers and optim's processing, misfit and preconditioning.
 
     obs: observed gathers (ns, ng, nt) as the solver writes them (host or device), or
     None to model them in simu_true."""
 
-    def __init__(self, simu, optim, obs=None, simu_true=None):
+    def __init__(self, simu, optim, obs=None, simu_true=None, lanes=2):
         from .engine import _require_gpu
         torch = _require_gpu()
         from scipy.integrate import cumulative_trapezoid
         from . import solver
-        from .engine import Plan
+        from .engine import Lanes
         inp = solver.solver_inputs(simu)
         self.nx, self.nz, self.nt, self.dt = inp["nx"], inp["nz"], inp["nt"], inp["dt"]
         self.ns, self.ng = simu.source.n, simu.receiver.n
@@ -55,9 +56,12 @@ class DeviceProblem:
         self.kind = optim.misfit_type
         if self.kind.lower() == 'rtm':
             raise ValueError("the device loop takes a misfit, not 'RTM'")
-        self.plan = Plan(inp["nx"], inp["nz"], inp["dx"], inp["dt"], inp["nt"], inp["npml"], inp["src_xz"],
-                         inp["rec_xz"], free_surface=inp["free_surface"], path="batched", strips=solver.STRIPS)
-        self.device = self.plan.device
+        # two lanes (shot halves on streams of their own): one lane's adjoint overlaps the
+        # other's forward
+        self.lanes = Lanes(inp["nx"], inp["nz"], inp["dx"], inp["dt"], inp["nt"], inp["npml"], inp["src_xz"],
+                           inp["rec_xz"], lanes=lanes, free_surface=inp["free_surface"], path="batched",
+                           strips=solver.STRIPS)
+        self.device = self.lanes.device
         src = cumulative_trapezoid(simu.source.wavelet, axis=-1, initial=0)
         src = np.array(np.broadcast_to(np.asarray(src, dtype=np.float32), (self.ns, self.nt)))
         self.src = torch.from_numpy(src).to(self.device)
@@ -73,41 +77,44 @@ class DeviceProblem:
                 raise ValueError("need observed gathers or simu_true")
             vt = torch.from_numpy(solver.solver_inputs(simu_true)["vp"]).to(self.device)
             rt = torch.from_numpy(solver.solver_inputs(simu_true)["rho"]).to(self.device)
-            self.plan.set_model(vt, rt)
-            obs = self.plan.forward(self.src, save_strips=0).clone()
+            self.lanes.set_model(vt, rt)
+            obs = self.lanes.forward(self.src, save_strips=0)
         self.obs = self._process(torch.as_tensor(np.asarray(obs) if not torch.is_tensor(obs) else obs,
                                                  dtype=torch.float32).to(self.device).contiguous().clone())
         # pick the step kernels' launch configuration once, in the starting model (the
         # results do not depend on it)
-        self.plan.set_model(torch.from_numpy(inp["vp"]).to(self.device), self.rho)
-        self.tuned = self.plan.autotune(steps=20)
+        self.lanes.set_model(torch.from_numpy(inp["vp"]).to(self.device), self.rho)
+        self.tuned = self.lanes.autotune(steps=20)
 
-    def _process(self, gathers):
+    def _process(self, gathers, offsets=None):
         if not self.identity:
-            preprocess.process_gathers_device(self.optim, gathers, self.dt, self.offsets)
+            preprocess.process_gathers_device(self.optim, gathers, self.dt,
+                                              self.offsets if offsets is None else offsets)
         return gathers
 
-    def _forward(self, v, save):
+    def _lane_process(self, i, r, syn):
+        """Processing of lane i's gathers (shots r[0]..r[1]-1), in place."""
+        if not self.identity:
+            self._process(syn, self.offsets[r[0]:r[1]])
+
+    def _set(self, v):
         torch = _torch()
-        self.plan.set_model(v.reshape(self.nx, self.nz).to(torch.float32), self.rho)
-        self.forwards += 1
-        return self._process(self.plan.forward(self.src, save_strips=1 if save else 0))
+        self.lanes.set_model(v.reshape(self.nx, self.nz).to(torch.float32), self.rho)
 
     def misfit(self, v):
         """Misfit of model vector v: forward without strips, processing, misfit."""
-        from .engine import misfit
-        syn = self._forward(v, False)
-        J, _, _ = misfit(self.kind, self.obs, syn, self.dt)
-        return float(J)
+        self._set(v)
+        self.forwards += 1
+        return float(self.lanes.misfit(self.src, self.obs, self.kind, self.dt, process=self._lane_process))
 
     def gradient(self, v):
         """(misfit, preconditioned gradient vector float64) of model vector v, as
         forward(savesnap=1), process_workflow, misfit, adjoint (with grad_precond)."""
         torch = _torch()
-        from .engine import grad_precond, misfit
-        syn = self._forward(v, True)
-        J, adj, _ = misfit(self.kind, self.obs, syn, self.dt)
-        g, fw, bk = self.plan.adjoint(self.src, adj)
+        from .engine import grad_precond
+        self._set(v)
+        self.forwards += 1
+        J, g, fw, bk = self.lanes.gradient(self.src, self.obs, self.kind, self.dt, process=self._lane_process)
         self.adjoints += 1
         o = self.optim
         # the solver's kernel and illuminations are float32 files (toolbox/solver.py:86-92)
@@ -117,7 +124,7 @@ class DeviceProblem:
         return float(J), gp.reshape(-1)
 
     def close(self):
-        self.plan.close()
+        self.lanes.close()
 
 
 # -- search directions (inversion.py:112-263) -----------------------------------------------
