@@ -231,7 +231,7 @@ def main():
     ap.add_argument("--cpu-shots", type=int, default=0, help="CPU baseline sample size (0: auto)")
     ap.add_argument("--plain-shots", type=int, default=2, help="plain-path comparison sample (0: skip)")
     ap.add_argument("--iters", type=int, default=10, help="L-BFGS iterations of --workload lbfgs")
-    ap.add_argument("--out", default="gpurun_out/precision_c5.npz",
+    ap.add_argument("--out", default="precision_c5.npz",
                     help="with --workload precision: where the two-shot gradients go")
     ap.add_argument("--workload", default="c2", choices=["c2", "uneven", "lbfgs", "precision"],
                     help="c2: the 481x141x4000, 49-shot gradient (default); uneven: three grid "
