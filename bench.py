@@ -519,9 +519,12 @@ def bench_precision(args):
 
 
 def bench_uneven(args):
-    """Three grid families in one plan group: each step is one grouped forward with
-    strips, the global-correlation misfit and adjoint source on the device, and one
-    grouped adjoint.  The same work run plan by plan is timed beside it."""
+    """Three grid families in one plan group: each step is one gradient of all 64 shots,
+    as a group forward with strips (every plan's time loop on a branch of one graph
+    launch), the global-correlation misfit and adjoint source on the device, and a group
+    adjoint.  Timed beside it: the whole gradient in one graph launch
+    (swit_group_gradient, where one plan's adjoint runs beside another's forward), and
+    the plans one after the other."""
     import torch
     from scipy.integrate import cumulative_trapezoid
     from swit_amd import workloads
@@ -546,33 +549,46 @@ def bench_uneven(args):
     models = [(torch.from_numpy(w.vp_init).to(dev), torch.from_numpy(w.rho(w.vp_init)).to(dev)) for w in ws]
     torch.cuda.synchronize()
 
-    def step(grouped):
+    def step(mode):
+        """one gradient of all 64 shots: 'gradient' = one graph launch per gradient
+        (swit_group_gradient), 'phases' = a group forward, the misfits, a group
+        adjoint, 'plans' = the plans one after the other"""
         for p, (vp, rho) in zip(plans, models):
             p.set_model(vp, rho)
-        if grouped:
+        if mode == "gradient":
+            Js, _, out = group.gradient(srcs, obs, "Globalcorrelation")
+            J = Js[0]
+            for x in Js[1:]:
+                J = J + x
+            return J, out
+        if mode == "phases":
             syn = group.forward(srcs, save_strips=True)
         else:
             syn = [p.forward(s, save_strips=True) for p, s in zip(plans, srcs)]
         res = [misfit("Globalcorrelation", o, s_, w.dt) for o, s_, w in zip(obs, syn, ws)]
         adj = [r[1] for r in res]
-        if grouped:
+        if mode == "phases":
             out = group.adjoint(srcs, adj)
         else:
             out = [p.adjoint(s, a) for p, s, a in zip(plans, srcs, adj)]
-        return sum(r[0] for r in res), out
+        J = res[0][0]
+        for r in res[1:]:
+            J = J + r[0]
+        return J, out
 
-    def timed(grouped):
+    def timed(mode):
         for _ in range(args.warmup):
-            step(grouped)
+            step(mode)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            J, _ = step(grouped)
+            J, _ = step(mode)
         torch.cuda.synchronize()
         return time.perf_counter() - t0, float(J)
 
-    el, J = timed(True)
-    el_seq, J_seq = timed(False)
+    el, J = timed("phases")
+    el_one, J_one = timed("gradient")
+    el_seq, J_seq = timed("plans")
     nshots = sum(w.ns for w in ws)
     line = {
         "metric": "shot-gradients/s, uneven 3-grid batch in one plan group (global-correlation misfit)",
@@ -581,8 +597,11 @@ def bench_uneven(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded layered-plus-fault models 1500-6000 m/s, Ricker 5 Hz)",
         "config": {"workload": "; ".join(f"{w.nx}x{w.nz} dx {w.dx} nt {w.nt} dt {w.dt}, {w.ns} shots" for w in ws),
-                   "path": "batched group", "strips": args.strips, "shots_per_step": nshots},
+                   "path": "batched group: group forward, misfits, group adjoint", "strips": args.strips,
+                   "shots_per_step": nshots},
         "misfit": J,
+        "one_launch": {"value": round(nshots * args.steps / el_one, 3), "ms_per_step": round(1000 * el_one / args.steps, 2),
+                       "misfit": J_one, "note": "whole gradient in one graph launch (swit_group_gradient)"},
         "plan_by_plan": {"value": round(nshots * args.steps / el_seq, 3), "ms_per_step": round(1000 * el_seq / args.steps, 2),
                          "misfit": J_seq, "group_over_plan_by_plan": round(el_seq / el, 3)},
     }

@@ -210,6 +210,20 @@ int swit_group_adjoint(swit_group *group, const float *const *src, const float *
                        hipStream_t stream);
 int swit_group_destroy(swit_group *group);
 
+/* A whole gradient of every plan in one graph launch: each plan's branch runs the
+ * strip-saving forward (seismograms to seis), the misfit and adjoint source `kind`
+ * (SWIT_MISFIT_*) against obs, exactly as swit_misfit computes them (adjoint source to
+ * adj, per-trace misfits to work, their sum to misfit), and the adjoint (grad,
+ * illum_forw, illum_back).  Each plan's results equal swit_forward, swit_misfit and
+ * swit_adjoint run one after the other.  The branches overlap, so one plan's adjoint
+ * runs beside another's forward.  Per plan: obs, seis, adj [ns][ngmax][nt]; work
+ * [ns*ngmax] doubles; misfit one double; grad, illum_* [nx][nz] doubles.  No counterpart in the
+ * reference (toolbox/inversion.py:48-60 runs these steps one after the other, through files). */
+int swit_group_gradient(swit_group *group, int kind, const float *const *src, const float *const *obs,
+                        float *const *seis, float *const *adj, double *const *work, double *const *misfit,
+                        double *const *grad, double *const *illum_forw, double *const *illum_back,
+                        hipStream_t stream);
+
 /* Bytes of device scratch the plan holds (wavefields, strips, accumulators). */
 int64_t swit_plan_scratch_bytes(const swit_plan *plan);
 

@@ -42,6 +42,7 @@ EXPORTS = (
     "swit_group_forward",
     "swit_group_adjoint",
     "swit_group_destroy",
+    "swit_group_gradient",
     "swit_plan_set_option",
     "swit_plan_kernel_time",
     "swit_grad_precond_work",
@@ -165,6 +166,8 @@ def load(path: str | None = None):
     L.swit_group_forward.restype = ctypes.c_int
     L.swit_group_adjoint.argtypes = [vp, pp, pp, pp, pp, pp, vp]
     L.swit_group_adjoint.restype = ctypes.c_int
+    L.swit_group_gradient.argtypes = [vp, ctypes.c_int, pp, pp, pp, pp, pp, pp, pp, pp, pp, vp]
+    L.swit_group_gradient.restype = ctypes.c_int
     L.swit_group_destroy.argtypes = [vp]
     L.swit_group_destroy.restype = ctypes.c_int
     L.swit_plan_strips.argtypes = [vp, vp, vp]

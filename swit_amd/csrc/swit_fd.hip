@@ -1200,6 +1200,10 @@ struct swit_group {
     hipGraphExec_t gf[2] = {nullptr, nullptr};   // forward without / with strip saving
     hipGraphExec_t gb = nullptr;
     std::vector<int> gens_f[2], gens_b;          // plan generations at capture
+    hipGraphExec_t gg = nullptr;                 // whole gradient (swit_group_gradient)
+    std::vector<int> gens_g;
+    std::vector<const void *> ptrs_g;            // the caller's arrays the gradient graph uses
+    int kind_g = -1;
 
     void drop(hipGraphExec_t &g) {
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
@@ -1208,6 +1212,7 @@ struct swit_group {
         drop(gf[0]);
         drop(gf[1]);
         drop(gb);
+        drop(gg);
         for (hipStream_t st : branch) (void)hipStreamDestroy(st);
         branch.clear();
         for (hipEvent_t e : join) (void)hipEventDestroy(e);
@@ -1409,16 +1414,19 @@ static void launch_fft_misfit(int kind, const FftFactors &f, int ntraces, float 
                            adj, work);
 }
 
-extern "C" int swit_misfit(int kind, int ntraces, int nt, float dt, const float *obs, const float *syn,
-                           float *adj, double *work, double *misfit, hipStream_t stream) {
-    g_err[0] = 0;
+static int misfit_check(int kind, int ntraces, int nt) {
     if (kind != SWIT_MISFIT_WAVEFORM && kind != SWIT_MISFIT_GLOBAL_CORRELATION && kind != SWIT_MISFIT_ENVELOPE &&
         kind != SWIT_MISFIT_TRAVELTIME)
         return set_err(SWIT_EINVAL, "unknown misfit kind %d", kind);
     if (ntraces < 1 || nt < 1) return set_err(SWIT_EINVAL, "need ntraces >= 1 and nt >= 1");
-    if (!obs || !syn || !adj || !work || !misfit) return set_err(SWIT_EINVAL, "all arrays are required");
     if ((kind == SWIT_MISFIT_ENVELOPE || kind == SWIT_MISFIT_TRAVELTIME) && nt > kFftMaxN)
         return set_err(SWIT_EINVAL, "envelope and traveltime misfits take nt <= %d (got %d)", kFftMaxN, nt);
+    return SWIT_OK;
+}
+
+// The misfit kernels of a checked call (also captured into group gradient graphs).
+static int launch_misfit(int kind, int ntraces, int nt, float dt, const float *obs, const float *syn, float *adj,
+                         double *work, double *misfit, hipStream_t stream) {
     if (kind == SWIT_MISFIT_WAVEFORM) {
         hipLaunchKernelGGL(k_misfit_trace<0>, dim3(ntraces), dim3(kMisfitThreads), 0, stream, nt, dt, obs, syn, adj,
                            work);
@@ -1434,6 +1442,105 @@ extern "C" int swit_misfit(int kind, int ntraces, int nt, float dt, const float 
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(kMisfitThreads), 0, stream, ntraces, work, misfit);
     LAUNCH_CHECK();
+    return SWIT_OK;
+}
+
+extern "C" int swit_misfit(int kind, int ntraces, int nt, float dt, const float *obs, const float *syn,
+                           float *adj, double *work, double *misfit, hipStream_t stream) {
+    g_err[0] = 0;
+    if (const int rc = misfit_check(kind, ntraces, nt)) return rc;
+    if (!obs || !syn || !adj || !work || !misfit) return set_err(SWIT_EINVAL, "all arrays are required");
+    return launch_misfit(kind, ntraces, nt, dt, obs, syn, adj, work, misfit, stream);
+}
+
+// ---- group gradient ---------------------------------------------------------------------
+
+// One plan's whole gradient on stream st: strip-saving forward, seismograms to seis,
+// misfit and adjoint source against obs (into adj, per-trace values in work, the sum in
+// misfit), the adjoint source to the time-major layout, adjoint, outputs to grad / fw / bk.
+static int plan_gradient_ops(swit_plan *pl, int kind, const float *obs, float *seis, float *adj, double *work,
+                             double *misfit, double *grad, double *fw, double *bk, hipStream_t st) {
+    const Dims &d = pl->d;
+    TRY(forward_prologue(pl, 1, st));
+    TRY(forward_loop(pl, 1, st));
+    TRY(forward_epilogue(pl, 1, st));
+    hipLaunchKernelGGL(k_transpose, dim3(cdiv(d.ngmax, 32), cdiv(d.nt, 32), pl->ns), dim3(32, 8), 0, st, d.nt,
+                       d.ngmax, pl->seis_t, seis);
+    LAUNCH_CHECK();
+    TRY(launch_misfit(kind, pl->ns * d.ngmax, d.nt, d.dt, obs, seis, adj, work, misfit, st));
+    hipLaunchKernelGGL(k_transpose, dim3(cdiv(d.nt, 32), cdiv(d.ngmax, 32), pl->ns), dim3(32, 8), 0, st, d.ngmax,
+                       d.nt, adj, pl->sadj_t);
+    LAUNCH_CHECK();
+    TRY(adjoint_prologue(pl, st));
+    TRY(adjoint_loop(pl, st));
+    TRY(adjoint_epilogue(pl, st));
+    const size_t ni = (size_t)d.nx * d.nz;
+    const double *o = pl->bat.out;
+    HIP_TRY(hipMemcpyAsync(grad, o, sizeof(double) * ni, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(fw, o + ni, sizeof(double) * ni, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(bk, o + 2 * ni, sizeof(double) * ni, hipMemcpyDeviceToDevice, st));
+    return SWIT_OK;
+}
+
+extern "C" int swit_group_gradient(swit_group *gr, int kind, const float *const *src, const float *const *obs,
+                                   float *const *seis, float *const *adj, double *const *work,
+                                   double *const *misfit, double *const *grad, double *const *illum_forw,
+                                   double *const *illum_back, hipStream_t stream) {
+    g_err[0] = 0;
+    if (!gr || !src || !obs || !seis || !adj || !work || !misfit || !grad || !illum_forw || !illum_back)
+        return set_err(SWIT_EINVAL, "group and every array list are required");
+    const size_t n = gr->plans.size();
+    std::vector<const void *> ptrs;
+    for (size_t i = 0; i < n; ++i) {
+        swit_plan *pl = gr->plans[i];
+        if (!src[i] || !obs[i] || !seis[i] || !adj[i] || !work[i] || !misfit[i] || !grad[i] || !illum_forw[i] ||
+            !illum_back[i])
+            return set_err(SWIT_EINVAL, "plan %zu: missing array", i);
+        if (!pl->model_set) return set_err(SWIT_ESTATE, "plan %zu: no model set", i);
+        TRY(misfit_check(kind, pl->ns * pl->d.ngmax, pl->d.nt));
+        for (const void *q : {(const void *)obs[i], (const void *)seis[i], (const void *)adj[i],
+                              (const void *)work[i], (const void *)misfit[i], (const void *)grad[i],
+                              (const void *)illum_forw[i], (const void *)illum_back[i]})
+            ptrs.push_back(q);
+    }
+    for (size_t i = 0; i < n; ++i) {
+        swit_plan *pl = gr->plans[i];
+        pl->strips_valid = false;
+        HIP_TRY(hipMemcpyAsync(pl->bat.src, src[i], sizeof(float) * pl->ns * pl->d.nt, hipMemcpyDeviceToDevice,
+                               stream));
+    }
+    // the graph holds the caller's arrays: recapture when they, the kind or a plan's
+    // launch configuration change
+    if (gr->gg && (gr->gens_g != gr->gens() || gr->ptrs_g != ptrs || gr->kind_g != kind)) gr->drop(gr->gg);
+    if (!gr->gg) {
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(gr->cap, hipStreamCaptureModeThreadLocal));
+        int rc = SWIT_OK;
+        hipError_t e = hipEventRecord(gr->fork, gr->cap);
+        for (size_t i = 0; i < n && rc == SWIT_OK && e == hipSuccess; ++i) {
+            hipStream_t st = gr->branch[i];
+            e = hipStreamWaitEvent(st, gr->fork, 0);
+            if (e != hipSuccess) break;
+            rc = plan_gradient_ops(gr->plans[i], kind, obs[i], seis[i], adj[i], work[i], misfit[i], grad[i],
+                                   illum_forw[i], illum_back[i], st);
+            if (rc == SWIT_OK) e = hipEventRecord(gr->join[i], st);
+            if (rc == SWIT_OK && e == hipSuccess) e = hipStreamWaitEvent(gr->cap, gr->join[i], 0);
+        }
+        hipError_t ee = hipStreamEndCapture(gr->cap, &graph);
+        if (rc != SWIT_OK || e != hipSuccess || ee != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            if (rc != SWIT_OK) return rc;
+            return set_err(SWIT_EHIP, "group capture: %s", hipGetErrorString(e != hipSuccess ? e : ee));
+        }
+        e = hipGraphInstantiate(&gr->gg, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return set_err(SWIT_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+        gr->gens_g = gr->gens();
+        gr->ptrs_g = ptrs;
+        gr->kind_g = kind;
+    }
+    HIP_TRY(hipGraphLaunch(gr->gg, stream));
+    for (size_t i = 0; i < n; ++i) gr->plans[i]->strips_valid = true;
     return SWIT_OK;
 }
 

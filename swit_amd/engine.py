@@ -250,6 +250,36 @@ class PlanGroup:
             self._ptrs([o[1] for o in outs]), self._ptrs([o[2] for o in outs]), self.plans[0]._stream()))
         return [tuple(o) for o in outs]
 
+    def gradient(self, srcs, obs, kind):
+        """Every plan's whole gradient in one graph launch (swit_group_gradient): forward
+        with strips, misfit `kind` and adjoint source against obs[i], adjoint.  Returns
+        (misfits, seismograms, (grad, illum_forw, illum_back) per plan); each plan's
+        results equal forward, engine.misfit and adjoint run one after the other.  The
+        returned tensors are the group's own and are overwritten by the next call."""
+        torch = self._torch
+        code = _MISFIT_KIND[kind.lower()]
+        ins = [p._dev(s, torch.float32, (p.ns, p.nt)) for p, s in zip(self.plans, srcs, strict=True)]
+        ob = [p._dev(o, torch.float32, (p.ns, p.ngmax, p.nt)) for p, o in zip(self.plans, obs, strict=True)]
+        # the graph holds the arrays' addresses: the outputs and work arrays are kept and
+        # reused from call to call (so the graph is captured once), and the observed
+        # gathers are held until the next call
+        if getattr(self, "_bufs", None) is None:
+            seis = [torch.empty((p.ns, p.ngmax, p.nt), dtype=torch.float32, device=p.device) for p in self.plans]
+            self._bufs = (seis, [torch.empty_like(x) for x in seis],
+                          [torch.empty(p.ns * p.ngmax, dtype=torch.float64, device=p.device) for p in self.plans],
+                          [torch.empty(1, dtype=torch.float64, device=p.device) for p in self.plans],
+                          [[torch.empty((p.nx, p.nz), dtype=torch.float64, device=p.device) for _ in range(3)]
+                           for p in self.plans])
+        seis, adj, work, J, outs = self._bufs
+        self._obs_held = ob
+        _native.check(self._lib.swit_group_gradient(
+            self._h, code, self._ptrs(ins), self._ptrs(ob), self._ptrs(seis), self._ptrs(adj), self._ptrs(work),
+            self._ptrs(J), self._ptrs([o[0] for o in outs]), self._ptrs([o[1] for o in outs]),
+            self._ptrs([o[2] for o in outs]), self.plans[0]._stream()))
+        for p, s in zip(self.plans, ins):
+            p._src = s
+        return [j[0] for j in J], seis, [tuple(o) for o in outs]
+
     def close(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

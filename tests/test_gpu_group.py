@@ -59,3 +59,44 @@ def test_group_matches_single_plans():
     grp.close()
     for p in plans:
         p.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["Waveform", "Globalcorrelation"])
+def test_group_gradient_equals_the_steps_one_by_one(kind):
+    """swit_group_gradient (one graph: forward, misfit and adjoint source, adjoint per
+    plan, plans on concurrent branches) gives each plan the bits of swit_forward,
+    swit_misfit and swit_adjoint run one after the other; a second call (the graph
+    replayed) gives the same bits again."""
+    import torch
+    from helpers import integrated_sources
+    from swit_amd import workloads
+    from swit_amd.engine import Plan, PlanGroup, misfit
+    ws = [workloads.small(seed=51, nx=50, nz=34, npml=10, nt=210, ns=3),
+          workloads.small(seed=52, nx=64, nz=28, npml=8, nt=160, ns=2, dx=12.0, dt=0.0012)]
+    plans, srcs, obs, ref = [], [], [], []
+    for w in ws:
+        src = torch.from_numpy(integrated_sources(w)).cuda()
+        p = Plan(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, w.src_xz, w.rec_xz)
+        p.set_model(torch.from_numpy(w.vp_true).cuda(), torch.from_numpy(workloads.gardner(w.vp_true)).cuda())
+        o = p.forward(src, save_strips=False).clone()
+        p.set_model(torch.from_numpy(w.vp_init).cuda(), torch.from_numpy(workloads.gardner(w.vp_init)).cuda())
+        syn = p.forward(src, save_strips=True)
+        J, adj, _ = misfit(kind, o, syn, w.dt)
+        g = p.adjoint(src, adj)
+        ref.append((float(J), syn.cpu().numpy(), [t.cpu().numpy() for t in g]))
+        plans.append(p)
+        srcs.append(src)
+        obs.append(o)
+    grp = PlanGroup(plans)
+    for _ in range(2):
+        Js, seis, outs = grp.gradient(srcs, obs, kind)
+        torch.cuda.synchronize()
+        for (J0, s0, g0), J, s, g in zip(ref, Js, seis, outs):
+            assert float(J) == J0
+            np.testing.assert_array_equal(s.cpu().numpy(), s0)
+            for a, b in zip(g, g0):
+                np.testing.assert_array_equal(a.cpu().numpy(), b)
+    grp.close()
+    for p in plans:
+        p.close()
