@@ -229,6 +229,9 @@ def main():
     ap.add_argument("--no-autograd", action="store_true",
                     help="skip the exact-adjoint (torch.autograd) gradient leg")
     ap.add_argument("--cpu-shots", type=int, default=0, help="CPU baseline sample size (0: auto)")
+    ap.add_argument("--no-lanes", action="store_true",
+                    help="skip the two-lane leg (its half-size launches of the same kernels would mix into a "
+                         "profiler's per-kernel averages)")
     ap.add_argument("--plain-shots", type=int, default=2, help="plain-path comparison sample (0: skip)")
     ap.add_argument("--iters", type=int, default=10, help="L-BFGS iterations of --workload lbfgs")
     ap.add_argument("--out", default="precision_c5.npz",
@@ -350,7 +353,8 @@ def main():
                     "time_loops_gbs": round((kern["forward"]["step_bytes"] * f_n + kern["backward"]["step_bytes"]
                                              * b_n) / ((f_ms + b_ms) * 1e-3) / 1e9, 1)}
 
-    lanes = lanes_leg(w, src, obs, vp_init, rho_init, g[0], args) if args.path == "batched" else None
+    lanes = (lanes_leg(w, src, obs, vp_init, rho_init, g[0], args)
+             if args.path == "batched" and not args.no_lanes else None)
     misfits = processing = None
     if args.path == "batched":
         syn_now = plan.forward(src, save_strips=False)
