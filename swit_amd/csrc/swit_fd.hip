@@ -233,7 +233,12 @@ static int plan_build(swit_plan *pl, const swit_grid *g, const swit_shots *sh, i
     if (!s16) TRY(pl->alloc(&pl->strips, (size_t)ns * d.nt * d.S));
     TRY(pl->alloc(&pl->seis_t, (size_t)ns * d.nt * ngmax));
     TRY(pl->alloc(&pl->sadj_t, (size_t)ns * d.nt * ngmax));
-    HIP_TRY(hipMemcpy(pl->geo, pl->geo_h.data(), sizeof(ShotGeom) * ns, hipMemcpyHostToDevice));
+    // Receiver columns past a shot's own count are never written by a kernel; they
+    // stay zero for the plan's life, so padded traces read as dead traces.
+    HIP_TRY(hipMemset(pl->seis_t, 0, sizeof(float) * ns * d.nt * ngmax));
+    HIP_TRY(hipMemset(pl->sadj_t, 0, sizeof(float) * ns * d.nt * ngmax));
+    HIP_TRY(hipStreamSynchronize(nullptr));
+    HIP_TRY(hipMemcpy(pl->geo,pl->geo_h.data(), sizeof(ShotGeom) * ns, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(pl->rec_fwd, rf.data(), sizeof(int) * rf.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(pl->rec_inj, ri.data(), sizeof(int) * ri.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(pl->strip_map, sm.data(), sizeof(uint32_t) * sm.size(), hipMemcpyHostToDevice));

@@ -10,17 +10,24 @@ from swit_amd import workloads
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("lanes,strips", [(2, "f32"), (3, "f16"), (1, "f32")])
-def test_lanes_match_one_plan(lanes, strips):
+@pytest.mark.parametrize("lanes,strips,ragged", [(2, "f32", False), (3, "f16", False), (1, "f32", False),
+                                                 (2, "f32", True)])
+def test_lanes_match_one_plan(lanes, strips, ragged):
     import torch
     from swit_amd.engine import Lanes, Plan, misfit
     w = workloads.small(seed=41, nx=70, nz=40, npml=10, nt=300, ns=5)
+    if ragged:
+        # fewer receivers on the last shots: the second lane's plan has a smaller ngmax
+        w.rec_xz = [w.rec_xz[i, :w.rec_xz.shape[1] - 9 * i] for i in range(w.ns)]
     src = torch.from_numpy(integrated_sources(w)).cuda()
     vt, rt = torch.from_numpy(w.vp_true).cuda(), torch.from_numpy(workloads.gardner(w.vp_true)).cuda()
     vi, ri = torch.from_numpy(w.vp_init).cuda(), torch.from_numpy(workloads.gardner(w.vp_init)).cuda()
     p = Plan(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, w.src_xz, w.rec_xz, strips=strips)
     p.set_model(vt, rt)
     obs = p.forward(src, save_strips=False).clone()
+    if ragged:                                         # padded receiver traces are dead (zero) traces
+        for i in range(w.ns):
+            assert not torch.any(obs[i, len(w.rec_xz[i]):])
     p.set_model(vi, ri)
     syn = p.forward(src, save_strips=True)
     J1, adj, _ = misfit("Waveform", obs, syn, w.dt)
