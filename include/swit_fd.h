@@ -89,7 +89,8 @@ int swit_plan_set_model(swit_plan *plan, const float *vp, const float *rho,
  * (solver.f90:36-116) / staggered42_modeling_is (fdcore.f90:30-198).
  *   src:  [ns][nt] integrated source traces (the reference's src%d.bin).
  *   seis: [ns][ngmax][nt] pressure seismograms (the SU trace data of src%d_sg.su),
- *         sample 0 is zero as in the reference.
+ *         sample 0 is zero as in the reference; traces past a shot's own receiver
+ *         count are zero (dead traces).
  *   save_strips: 1 keeps boundary strips and the final state on the device for a
  *         following swit_adjoint (the reference's STORE_SNAP=1 files,
  *         fdcore.f90:130-176); 0 invalidates them (the reference's cleandata of
