@@ -120,7 +120,8 @@ const char *swit_last_error(void);
  * batched path). */
 #define SWIT_OPT_GRAPH 1        /* 1 (default): replay the time loop as a hipGraph    */
 #define SWIT_OPT_PROFILE 2      /* 1: time each step-kernel time loop with HIP events */
-#define SWIT_OPT_FWD_GROUP 4    /* shots per workgroup of the forward step kernel     */
+#define SWIT_OPT_FWD_GROUP 4    /* G: the forward step kernel's workgroups take
+                                   ceil(ns/G) near-equal groups of at most G shots     */
 #define SWIT_OPT_FWD_COLGROUP 8 /* tile columns per group of the forward work order   */
 #define SWIT_OPT_BWD_COLGROUP 16 /* tile columns per group of the backward work order */
 #define SWIT_OPT_BWD_STEPS 32   /* 2 (default): backward kernel doing two time steps per
@@ -134,7 +135,7 @@ int swit_plan_set_option(swit_plan *plan, int option, int value);
 
 /* Autotuner (no counterpart in the reference): times `steps` back-to-back launches of
  * each batched step kernel (best of three bursts) on this plan's grid for every candidate forward
- * shots-per-workgroup (1..8) x column-group order (forward with strip saving) and
+ * shot group size (1..8, 10, 12) x column-group order (forward with strip saving) and
  * every backward kernel (two steps per launch, or one step per launch at each tile
  * width) x column-group order, and keeps the fastest per time step.  Call between jobs
  * with the model set; it uses the wavefield and strip buffers as scratch, so saved

@@ -303,14 +303,17 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
     __shared__ float sW[kNR];
     __shared__ float sQ[kNR];     // new p
 
+    // the ns shots in ceil(ns / G) groups of near-equal size (at most G), so that every
+    // workgroup of the launch carries about the same work
+    const int ngr = (ns + G - 1) / G;
     int grp, t;
-    work_item(xcd_logical(blockIdx.x, gridDim.x), (ns + G - 1) / G, tm, grp, t);
+    work_item(xcd_logical(blockIdx.x, gridDim.x), ngr, tm, grp, t);
     const TileBox tb = tile_box(d, tm, t);
     const int zo = tb.z0 - kHalo, xo = tb.x0 - kHalo;
     const int tid = threadIdx.x;
     const int fs = d.fs;
     const int np = d.nxp * d.ld;
-    const int s_begin = grp * G, s_end = min(ns, s_begin + G);
+    const int s_begin = grp * ns / ngr, s_end = (grp + 1) * ns / ngr;
 
     // staged fields of the first shot
     float lp[kLPT], lu[kLPT], lw[kLPT];
@@ -1078,7 +1081,7 @@ __global__ void k_end_state(Dims d, const float *__restrict__ S, float *__restri
 // Per-plan state of the batched path.
 struct BatchedState {
     int ready = 0;
-    int Gf = 1;                  // shots per forward workgroup
+    int Gf = 1;                  // at most this many shots per forward workgroup
     int ngroups_f = 0;
     TileMap fwd{}, bwd{};
     int *rec_start = nullptr;    // [ns][fwd.ntiles+1]

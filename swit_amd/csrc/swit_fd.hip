@@ -1069,7 +1069,7 @@ extern "C" int swit_plan_autotune(swit_plan *pl, int steps, int *choice, double 
     // first, and the saved strips are overwritten
     pl->strips_valid = false;
     bs.exact_valid = false;
-    static const int kGroups[] = {1, 2, 3, 4, 5, 6, 8};
+    static const int kGroups[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12};
     static const int kCols[] = {1, 2, 4, 8, 16};
     constexpr int kRepeats = 3;      // each candidate keeps its best of three bursts
     double best_f = 1e30, best_b = 1e30;
