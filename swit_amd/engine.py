@@ -295,9 +295,10 @@ class PlanGroup:
 
 class Lanes:
     """The shots of one job split into `lanes` batched plans (contiguous shot ranges),
-    each running its whole job on a stream of its own.  A gradient on two lanes lets
-    one lane's adjoint overlap the other's forward: at 481x141x4000 with 49 shots it runs
-    in 473 ms against 514 ms for one plan (three or four lanes were slower).  Gradients
+    each running its whole job on a stream of its own.  The lanes' step kernels run side
+    by side and fill each other's idle workgroup slots: at 481x141x4000 with 49 shots a
+    gradient on two lanes runs in 473-480 ms against 511-514 ms for one plan (three or
+    four lanes, or a second lane started after the first's forward, were slower).  Gradients
     and illuminations are the lanes' FP64 sums added in lane order, so they equal the
     one-plan result to FP64 rounding.  Plan keyword arguments (strips, ...) apply to
     every lane."""
