@@ -375,12 +375,17 @@ def main():
         pp.set_model(vp_true, rho_true)
         obs_k = pp.forward(src[:k], save_strips=False).clone()
         torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        pp.set_model(vp_init, rho_init)
-        syn_k = pp.forward(src[:k], save_strips=True)
-        pp.adjoint(src[:k], syn_k - obs_k)
-        torch.cuda.synchronize()
-        pv = k / (time.perf_counter() - t1)
+        # two gradients, the faster one kept: the first also pays the adjoint path's
+        # first-call costs
+        best = float("inf")
+        for _ in range(2):
+            t1 = time.perf_counter()
+            pp.set_model(vp_init, rho_init)
+            syn_k = pp.forward(src[:k], save_strips=True)
+            pp.adjoint(src[:k], syn_k - obs_k)
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t1)
+        pv = k / best
         pp.close()
         plain = {"value": round(pv, 3), "unit": "shot-gradients/s", "shots": k,
                  "batched_over_plain": round(value / pv, 2)}
