@@ -106,8 +106,8 @@ def autograd_leg(w, src, obs, vp_init, rho_init, args):
 
 def lanes_leg(w, src, obs, vp, rho, g_ref, args, lanes=2):
     """The same gradient on engine.Lanes: the shots split into `lanes` plans, each
-    running forward, adjoint source and adjoint on a stream of its own, so that one
-    lane's adjoint overlaps another's forward.  Timed as the main steps (warm-up, then
+    running forward, adjoint source and adjoint on a stream of its own, so that the
+    lanes' step kernels run side by side.  Timed as the main steps (warm-up, then
     `steps` gradients between synchronisations); the gradient is checked against the
     main step's (FP64 sums in another order)."""
     import torch

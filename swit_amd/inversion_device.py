@@ -56,8 +56,8 @@ class DeviceProblem:
         self.kind = optim.misfit_type
         if self.kind.lower() == 'rtm':
             raise ValueError("the device loop takes a misfit, not 'RTM'")
-        # two lanes (shot halves on streams of their own): one lane's adjoint overlaps the
-        # other's forward
+        # two lanes (shot halves on streams of their own): their step kernels run side by
+        # side and fill each other's idle workgroup slots
         self.lanes = Lanes(inp["nx"], inp["nz"], inp["dx"], inp["dt"], inp["nt"], inp["npml"], inp["src_xz"],
                            inp["rec_xz"], lanes=lanes, free_surface=inp["free_surface"], path="batched",
                            strips=solver.STRIPS)
