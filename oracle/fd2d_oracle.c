@@ -18,8 +18,11 @@
  *   orc_adjoint_shot   known answer ratio = 1.00000 (same notebook), reconstruction
  *                      invariant (reconstructed p == forward p at every step),
  *                      forward illumination == sum of forward p^2, kernel against the
- *                      finite-difference derivative of the L2 misfit (sign, dt scaling)
- *                      and against the exact discrete gradient (correlation)
+ *                      finite-difference derivative of the L2 misfit (sign, dt scaling,
+ *                      and <g, dc> = dt dJ/dc in absolute terms, which fixes the -2/c),
+ *                      against the exact discrete gradient (correlation), and the
+ *                      coefficient swap: in a heterogeneous medium the adjoint record
+ *                      equals the forward record of the medium (c, 1/(den c^2))
  *   orc_setup_pml      closed form of the quadratic profile, corner split, symmetry
  *   orc_pad, orc_vmin  brute force on small arrays
  * Parity unpinned: the density node used by the velocity updates in a heterogeneous

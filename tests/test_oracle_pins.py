@@ -14,6 +14,11 @@ sign, index or time level) fails one of them.
   * gradient kernel (solver.f90:237-270): proportional to dt, invariant to a uniform
     density scale, same sign as and correlated with the exact discrete gradient
     (torch autograd through a vectorised restatement checked against the oracle);
+    with the reconstruction exact and no boundary in reach, <g, dc> equals dt times
+    the finite-difference derivative (fixes the -2 and the 1/c of solver.f90:270);
+  * adjoint coefficient placement (fdcore.f90:310-345 against :77-113): in a
+    heterogeneous medium the adjoint step is the forward scheme of the medium
+    (c, 1/(den c^2)), so its record equals that forward run's seismograms;
   * one-step impulse response against the textbook 4th-order staggered stencil
     (c1 = 9/8, c2 = -1/24) and B = dt/(rho dx);
   * homogeneous-medium pressure against the exact 2-D Green's-function solution
@@ -369,3 +374,89 @@ def test_homogeneous_green_function():
         corr = max(np.dot(np.roll(num, -sh), pan) / np.linalg.norm(num) / np.linalg.norm(pan) for sh in (-1, 0))
         assert corr > 0.9995, (r, corr)
         assert abs(np.abs(num).max() / np.abs(pan).max() - 1) < 5e-3
+
+
+# ---------------------------------------------------------------------------------
+def test_adjoint_is_forward_of_the_swapped_medium():
+    """The adjoint step updates velocities with den c^2 dt/dx and pressure with
+    dt/(den dx) (fdcore.f90:310-345), the forward the other way round (:77-113).  Relabel
+    the adjoint's velocity after its next velocity update as the current one: the
+    adjoint is then the forward leapfrog of the medium (c, den' = 1/(den c^2)), whose
+    pressure coefficient den' c^2 dt/dx = dt/(den dx) and velocity coefficient
+    dt/(den' dx) = den c^2 dt/dx.  Adjoint source sadj(it) at reverse step it is
+    forward source s'(nt+2-it); the adjoint pressure after step it is the forward
+    record at nt+2-it.  With the source, the receivers and every wave inside the
+    interior for the whole record (neither the free-surface rows, where the two
+    sweeps differ, nor the absorbing layer are reached), the two agree to FP64
+    round-off in a heterogeneous medium, where an adjoint without the swap (the
+    original medium's forward) differs by the density contrast."""
+    from scipy.ndimage import gaussian_filter
+    nx = nz = 120
+    npml, dx, dt, nt = 10, 10.0, 0.001, 300
+    rng = np.random.default_rng(7)
+    vp = np.clip(2500 + 7000 * gaussian_filter(rng.standard_normal((nx, nz)), 4), 1900, 3100)
+    vp = vp.astype(np.float32)
+    rho = (310 * np.power(vp.astype(np.float64), 0.25)
+           * (1 + 2.0 * gaussian_filter(rng.standard_normal((nx, nz)), 3))).astype(np.float32)
+    assert rho.min() > 0 and rho.max() / rho.min() > 1.5
+    src = _src(nt, dt, 25.0)
+    sxz = np.array([[60 * dx, 60 * dx]], np.float32)
+    rxz = np.array([[[60, 60], [50, 62], [70, 55], [60, 48], [66, 71]]], np.float32) * dx
+    ng = rxz.shape[1]
+    m = O.Model(vp, rho, dx, npml, "f64")
+    f = O.forward(m, nt, dt, sxz, rxz, src[None].astype(np.float32))
+    adj = np.zeros((1, ng, nt), np.float32)
+    adj[0, 0] = src
+    rec = O.adjoint(m, nt, dt, sxz, rxz, src[None].astype(np.float32), adj, f, record=True)["adj_rec"][0]
+
+    j = np.arange(1, nt)                      # forward slot j = it'-1 <-> adjoint slot nt-j
+    s_rev = np.zeros(nt)
+    s_rev[j] = adj[0, 0][nt - j]
+    rec_rev = np.zeros_like(rec)
+    rec_rev[:, j] = rec[:, nt - j]
+    rho_sw = 1.0 / (rho.astype(np.float64) * vp.astype(np.float64) ** 2)
+    m_sw = O.Model(vp.astype(np.float64), rho_sw, dx, npml, "f64", exact_input=True)
+    fwd_sw = O.forward(m_sw, nt, dt, sxz, rxz, s_rev[None], save_strips=False)["seis"][0]
+    scale = np.abs(fwd_sw).max()
+    assert scale > 0
+    assert np.abs(rec_rev - fwd_sw).max() <= 1e-12 * scale
+    # control: the forward of the original medium (an adjoint without the swap)
+    fwd_plain = O.forward(m, nt, dt, sxz, rxz, s_rev[None], save_strips=False)["seis"][0]
+    assert np.abs(rec_rev - fwd_plain).max() > 1e-2 * scale
+
+
+@pytest.mark.parametrize("dt", [0.001, 0.0005])
+def test_gradient_kernel_absolute_scale(dt):
+    """g = -2/c sum_t p_rec (p0_bk - p_bk) (solver.f90:237,270).  In a homogeneous medium
+    the swapped adjoint is the exact transpose, its pressure is dt times the exact
+    adjoint variable (injection weight dt, fdcore.f90:353), and -(2/c) sum p dlambda is
+    the summation by parts of sum lambda (dK/dc) (-D v) with dK/dc = 2K/c.  So
+    <g, dc> = dt dJ/dc for J = 1/2 |d - d_obs|^2, provided the reconstruction is exact
+    (source at depth 2 cells, as in every benchmark geometry; a deeper source takes the
+    s(it+1) path of fdcore.f90:250-257) and no wave that reaches the free surface or
+    the absorbing layer returns to the receivers within the record.  The remaining
+    difference is the half-step offset between p_rec and the adjoint difference:
+    1.6 % at dt = 1 ms, 0.5 % at 0.5 ms.  A -1/c scale gives 0.5, a sign error -1."""
+    nx, nz, npml, dx = 120, 100, 10, 10.0
+    nt = int(round(0.3 / dt))
+    c = np.full((nx, nz), 2500.0)
+    rho = np.full((nx, nz), 2000.0, np.float32)
+    X, Z = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+    dc = np.exp(-((X - 60) ** 2 + (Z - 30) ** 2) / (2 * 3.0 ** 2))
+    src = _src(nt, dt, 20.0)[None].astype(np.float32)
+    sxz = np.array([[60 * dx, 2 * dx]], np.float32)
+    xs = np.arange(45, 76)
+    rxz = np.stack([xs * dx, np.full(len(xs), 45 * dx)], 1)[None].astype(np.float32)
+
+    def seis(cc, save=False):
+        m = O.Model(cc, rho, dx, npml, "f64", exact_input=True)
+        return m, O.forward(m, nt, dt, sxz, rxz, src, save_strips=save)
+    dobs = seis(c + 20 * dc)[1]["seis"]
+    m, f = seis(c, True)
+    r = f["seis"] - dobs
+    g = O.adjoint(m, nt, dt, sxz, rxz, src, r, f)["grad"]
+    eps = 0.5
+    jp = 0.5 * np.sum((seis(c + eps * dc)[1]["seis"] - dobs) ** 2)
+    jm = 0.5 * np.sum((seis(c - eps * dc)[1]["seis"] - dobs) ** 2)
+    ratio = np.sum(g * dc) / ((jp - jm) / (2 * eps)) / float(np.float32(dt))
+    assert abs(ratio - 1) < (0.025 if dt == 0.001 else 0.008), ratio
