@@ -111,3 +111,14 @@ def torch_forward(vp, rho, dx, dt, npml, src_xz, rec_xz, src, damp, free_surface
             rec.append(p[rz, rx])
         out.append(torch.stack(rec, 1))
     return torch.stack(out, 0)
+
+
+def oracle_threads():
+    """Threads for the CPU oracle: the CPUs this process may run on, at most 16 (the
+    host share of one GPU on the test machines)."""
+    import os
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))

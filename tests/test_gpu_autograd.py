@@ -1,9 +1,10 @@
 """Exact adjoint behind torch.autograd (swit_amd.autograd): the seismograms equal the
 ordinary forward's bit for bit; the gradient equals torch autograd through an FP64
 restatement of the same scheme (tolerance: relative L2 1e-4, FP32 against FP64); and
-on the 161x61, 8-shot, 1500-step case the directional derivative <g, dm> matches FP64
-central finite differences of the FP64 oracle at eps = 1e-2 .. 1e-5 (tolerance 1e-3
-relative) for the L2 and the global-correlation misfits."""
+the directional derivative <g, dm> matches FP64 central finite differences of the FP64
+oracle at eps = 1e-2 .. 1e-5 (tolerance 1e-3 relative) for the L2 and the
+global-correlation misfits, on the 161x61, 8-shot, 1500-step case (whole history and
+checkpoints) and on two shots of the 481x141, 4000-step benchmark grid (checkpoints)."""
 import numpy as np
 import pytest
 
@@ -62,37 +63,76 @@ def test_exact_gradient_matches_autograd_restatement(free_surface):
     p.close()
 
 
-@pytest.mark.gpu
-def test_exact_gradient_finite_differences_161x61():
+def _finite_difference_check(w, dm, mode, sel=None):
+    """<g, dm> of the exact adjoint (FP32 kernels, plan mode `mode`) against FP64 central
+    finite differences of the FP64 oracle along dm at eps = 1e-2 .. 1e-5, for the L2 and
+    global-correlation misfits; relative error <= 1e-3.  The damping profile is frozen
+    at the unperturbed model's (the exact adjoint holds it fixed)."""
     import torch
+    from helpers import oracle_threads
     from swit_amd.autograd import global_correlation_misfit, l2_misfit, seismograms
-    w = workloads.config3()
-    src = integrated_sources(w)
+    from swit_amd.engine import Plan
+    sel = slice(None) if sel is None else sel
+    sxz, rxz = w.src_xz[sel], w.rec_xz[sel]
+    src = integrated_sources(w)[sel]
     rho = workloads.gardner(w.vp_init)
-    dm = w.extra["dm"].astype(np.float64)
-    p = _plan(w)
+    p = Plan(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, sxz, rxz, path="batched", exact=mode)
     p.set_model(torch.from_numpy(w.vp_true), torch.from_numpy(workloads.gardner(w.vp_true)))
     obs = p.forward_exact(torch.from_numpy(src)).clone()
     obs_np = obs.cpu().numpy().astype(np.float64)
     vp0 = w.vp_init.astype(np.float32)
     m0 = O.Model(vp0, rho, w.dx, w.npml, "f64")
-
-    def J_oracle(vp, kind):
-        m = O.Model(vp, rho, w.dx, w.npml, "f64", damp_override=m0.damp, exact_input=True)
-        s = O.forward(m, w.nt, w.dt, w.src_xz, w.rec_xz, src, save_strips=False)["seis"]
-        st, ot = torch.from_numpy(s), torch.from_numpy(obs_np)
-        return float(l2_misfit(st, ot) if kind == "l2" else global_correlation_misfit(st, ot))
-
-    for kind, fn in (("l2", l2_misfit), ("gc", global_correlation_misfit)):
+    th = oracle_threads()
+    fns = {"l2": l2_misfit, "gc": global_correlation_misfit}
+    gd = {}
+    for kind, fn in fns.items():
         vp = torch.tensor(vp0, device="cuda", requires_grad=True)
         seis = seismograms(vp, p, torch.from_numpy(src).cuda(), torch.from_numpy(rho).cuda())
         fn(seis, obs).backward()
-        gd = float(np.sum(vp.grad.double().cpu().numpy() * dm))
-        for eps in (1e-2, 1e-3, 1e-4, 1e-5):
-            fd = (J_oracle(vp0 + eps * dm, kind) - J_oracle(vp0 - eps * dm, kind)) / (2 * eps)
-            print(f"{kind} eps={eps:g}: <g,dm> {gd:.9e}  FD {fd:.9e}  rel {abs(gd - fd) / abs(fd):.2e}")
-            assert abs(gd - fd) <= 1e-3 * abs(fd)
+        gd[kind] = float(np.sum(vp.grad.double().cpu().numpy() * dm))
     p.close()
+
+    def J_oracle(vp):
+        m = O.Model(vp, rho, w.dx, w.npml, "f64", damp_override=m0.damp, exact_input=True)
+        s = torch.from_numpy(O.forward(m, w.nt, w.dt, sxz, rxz, src, save_strips=False, threads=th)["seis"])
+        ot = torch.from_numpy(obs_np)
+        return {k: float(fn(s, ot)) for k, fn in fns.items()}
+
+    worst = 0.0
+    for eps in (1e-2, 1e-3, 1e-4, 1e-5):
+        jp, jm = J_oracle(vp0 + eps * dm), J_oracle(vp0 - eps * dm)
+        for kind in fns:
+            fd = (jp[kind] - jm[kind]) / (2 * eps)
+            rel = abs(gd[kind] - fd) / abs(fd)
+            worst = max(worst, rel)
+            print(f"{w.nx}x{w.nz}x{w.nt} {mode} {kind} eps={eps:g}: <g,dm> {gd[kind]:.9e}  FD {fd:.9e}  "
+                  f"rel {rel:.2e}", flush=True)
+            assert rel <= 1e-3, (kind, eps, rel)
+    return worst
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [True, "checkpoint"])
+def test_exact_gradient_finite_differences_161x61(mode):
+    """The autograd check configuration: 161x61, nt 1500, 8 shots, smooth perturbation
+    scaled to 50 m/s, with the whole divergence history and with checkpoints."""
+    w = workloads.config3()
+    _finite_difference_check(w, w.extra["dm"].astype(np.float64), mode)
+
+
+@pytest.mark.gpu
+def test_exact_gradient_finite_differences_481x141x4000():
+    """The benchmark's grid and record length (481x141, npml 40, nt 4000), shots 0 and
+    24, a smooth random perturbation (N(0,1) noise Gaussian-filtered with sigma 6 cells,
+    scaled to 50 m/s); checkpointed sweep, as the benchmark's autograd leg runs it when
+    the whole history does not fit."""
+    from scipy.ndimage import gaussian_filter
+    w = workloads.config2()
+    rng = np.random.default_rng(22)
+    dm = gaussian_filter(rng.standard_normal((w.nx, w.nz)), 6)
+    dm = dm / np.abs(dm).max() * 50.0
+    worst = _finite_difference_check(w, dm, "checkpoint", sel=[0, 24])
+    print(f"481x141x4000: worst relative error of <g,dm> against FP64 finite differences {worst:.2e}")
 
 
 @pytest.mark.gpu
