@@ -240,16 +240,13 @@ def main():
                     help="c2: the 481x141x4000, 49-shot gradient (default); uneven: three grid "
                          "families (501x171x5000, 481x121x4000, 301x101x3000; 64 shots) as one plan "
                          "group with the global-correlation misfit on the device; lbfgs: --iters "
-                         "L-BFGS iterations of the reference inversion loop through the drop-in solver "
+                         "L-BFGS iterations of the device-resident inversion loop "
                          "on the 801x267x6000, 81-shot model with 16-bit strips (--strips, default f16); "
                          "precision: two shots of that model with FP32 and 16-bit strips, gradients to --out "
                          "for tests/precision_c5.py")
     ap.add_argument("--bwd-steps", type=int, default=0, choices=[0, 1, 2],
                     help="reverse steps per backward launch (1 or 2) after autotuning; 0: the autotuner's "
                          "choice")
-    ap.add_argument("--resident", action="store_true",
-                    help="with --workload lbfgs: run the device-resident loop (swit_amd.inversion_device) "
-                         "instead of the file-based one")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (5 times): the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -427,13 +424,13 @@ def main():
 
 
 def bench_lbfgs(args):
-    """L-BFGS through the reference loop (swit_amd.inversion, toolbox/inversion.py) and
-    the drop-in solver, SU files in a RAM-backed folder; 16-bit strips, FP64 shot sum.
-    (The error the 16-bit strips add is measured against the FP64 oracle by
-    tests/test_gpu_strips.py.)"""
+    """L-BFGS iterations of the device-resident inversion loop (swit_amd.inversion_device,
+    the reference loop's decisions with every vector on the device) on the 801x267x6000,
+    81-shot model, 16-bit strips (--strips, default f16), FP64 shot sum.  The observed
+    data are modelled on the device in the true model first (timed apart)."""
     import shutil
     import tempfile
-    from swit_amd import base, inversion, preprocess, solver, workloads
+    from swit_amd import base, inversion_device, solver, workloads
     strips = args.strips if args.strips != "f32" else "f16"
     w = workloads.config5()
     root = "/dev/shm" if os.path.isdir("/dev/shm") else None
@@ -450,43 +447,26 @@ def bench_lbfgs(args):
                           1.0, 10.0, False, 0.0, 'None', False, False, 0.0, 1e9)
     solver.STRIPS = strips
     init = {'vp': w.vp_init.astype(np.float64), 'rho': w.rho(w.vp_init).astype(np.float64)}
-    if args.resident:
-        # device-resident loop: observed data modelled and kept on the device
-        from swit_amd import inversion_device
-        t0 = time.perf_counter()
-        problem = inversion_device.DeviceProblem(simulate(w.vp_init), optim, simu_true=simulate(w.vp_true))
-        t_obs = time.perf_counter() - t0
-        simu = simulate(w.vp_init)
-        t0 = time.perf_counter()
-        inversion_device.inversion(simu, optim, init, problem=problem)
-        el = time.perf_counter() - t0
-        problem.close()
-    else:
-        t0 = time.perf_counter()
-        simu_true = simulate(w.vp_true)
-        solver.forward(simu_true, 'obs', savesnap=0)
-        preprocess.process_workflow(simu_true, optim, 'obs')
-        t_obs = time.perf_counter() - t0
-        simu = simulate(w.vp_init)
-        t0 = time.perf_counter()
-        inversion.inversion(simu, optim, init)
-        el = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    problem = inversion_device.DeviceProblem(simulate(w.vp_init), optim, simu_true=simulate(w.vp_true))
+    t_obs = time.perf_counter() - t0
+    simu = simulate(w.vp_init)
+    t0 = time.perf_counter()
+    inversion_device.inversion(simu, optim, init, problem=problem)
+    el = time.perf_counter() - t0
+    problem.close()
     f = np.loadtxt(home + 'outputs/misfit_data.dat')
     steps = np.loadtxt(home + 'outputs/line_search_iteration.dat').reshape(-1)
-    solver.release()
     shutil.rmtree(home, ignore_errors=True)
 
     line = {
-        "metric": "L-BFGS iterations through the reference inversion loop, 801x267x6000, 81 shots"
-                  + (", device-resident vectors" if args.resident else ""),
+        "metric": "L-BFGS iterations of the device-resident inversion loop, 801x267x6000, 81 shots",
         "value": round(args.iters / el * 60.0, 3), "unit": "iterations/min", "n_gpus": 1,
         "steps": args.iters, "warmup": 0, "ms_per_step": round(1000 * el / args.iters, 1),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded folded layers 2000-6000 m/s, Ricker 5 Hz)",
         "config": {"workload": f"{w.nx}x{w.nz} dx {w.dx} npml {w.npml} nt {w.nt} dt {w.dt}, {w.ns} shots, "
-                               f"{w.ng} receivers, waveform misfit, "
-                               + ("device-resident loop (no SU files)" if args.resident
-                                  else "drop-in solver with SU files"),
+                               f"{w.ng} receivers, waveform misfit, device-resident loop (no SU files)",
                    "strips": strips, "cross_shot_sum": "f64"},
         "misfit": [float(x) for x in f], "line_search_forwards": [int(x) for x in steps],
         "observed_data_s": round(t_obs, 2),

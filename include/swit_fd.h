@@ -46,6 +46,10 @@ extern "C" {
                                    of history, about 4 sqrt(nt) * ns * (nx+2npml) * ld
                                    floats; the reverse sweep recomputes each segment's
                                    forward (same gradient, bit for bit) */
+#define SWIT_PLAN_NO_STRIPS 0x100 /* allocate no boundary strips: for plans that only model
+                                   (observed data, line-search forwards, the exact
+                                   adjoint); a strip-saving forward and swit_adjoint are
+                                   refused, the autotuner tunes the forward only */
 
 /* Grid and time axis: the reference's param type (fd2dmpi/datatype.f90:17-22) as
  * read from the parfile (io.f90:72-96).  dx and dt are the float32 values the
@@ -81,7 +85,11 @@ int swit_plan_create(const swit_grid *grid, const swit_shots *shots, int flags,
 /* Loads vp/rho ([nx][nz] float32, device) into the plan: PML padding
  * (io.f90:231-300), vmin (math.f90:342-361, solver.f90:73), damping profile
  * (pml.f90:74-101), and the per-cell coefficients den*c^2*dt/dx and dt/(den*dx)
- * (fdcore.f90:79,99,108).  Invalidates saved strips. */
+ * (fdcore.f90:79,99,108).  Invalidates saved strips.  Refuses (SWIT_EINVAL, no model
+ * loaded) a model with a vp or rho value that is not finite and positive, or whose
+ * largest vp breaks the 4th-order stability limit dt < sqrt(3/8) dx / max(vp) that
+ * toolbox/base.py:57-59 enforces; the check reads two values back, so the call waits
+ * for `stream`. */
 int swit_plan_set_model(swit_plan *plan, const float *vp, const float *rho,
                         hipStream_t stream);
 
@@ -228,6 +236,13 @@ int swit_group_gradient(swit_group *group, int kind, const float *const *src, co
 
 /* Bytes of device scratch the plan holds (wavefields, strips, accumulators). */
 int64_t swit_plan_scratch_bytes(const swit_plan *plan);
+
+/* Generation of the divergence history an exact plan holds: a number that each
+ * swit_forward_exact raises, or 0 while no valid history is held (none yet, or a new
+ * model or the autotuner has invalidated it).  A caller that keeps a forward's
+ * generation and compares it before swit_adjoint_exact knows that the history still
+ * belongs to that forward (swit_amd.autograd does).  No counterpart in the reference. */
+int64_t swit_plan_exact_generation(const swit_plan *plan);
 
 /* ---- gradient preconditioning (toolbox/postprocess.py:20-111) --------------------- */
 

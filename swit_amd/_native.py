@@ -23,6 +23,7 @@ SWIT_STRIPS_F16 = 0x10
 SWIT_STRIPS_BF16 = 0x20
 SWIT_PLAN_EXACT = 0x40
 SWIT_PLAN_EXACT_CKPT = 0x80
+SWIT_PLAN_NO_STRIPS = 0x100
 
 # Every entry point include/swit_fd.h declares.
 EXPORTS = (
@@ -33,6 +34,7 @@ EXPORTS = (
     "swit_plan_destroy",
     "swit_last_error",
     "swit_plan_scratch_bytes",
+    "swit_plan_exact_generation",
     "swit_plan_strips",
     "swit_plan_autotune",
     "swit_forward_exact",
@@ -174,6 +176,8 @@ def load(path: str | None = None):
     L.swit_plan_strips.restype = ctypes.c_int
     L.swit_plan_scratch_bytes.argtypes = [vp]
     L.swit_plan_scratch_bytes.restype = ctypes.c_int64
+    L.swit_plan_exact_generation.argtypes = [vp]
+    L.swit_plan_exact_generation.restype = ctypes.c_int64
     L.swit_plan_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.swit_plan_set_option.restype = ctypes.c_int
     L.swit_plan_kernel_time.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),

@@ -10,7 +10,9 @@ The gradient is that of the discrete scheme the forward runs (4th-order staggere
 velocity-pressure, fdcore.f90:75-128), through K = den c^2 dt/dx and the PML padding
 of the model (io.f90:231-300); density and the damping profile (including its vmin)
 are held fixed.  The plan keeps one divergence history, so a backward belongs to the
-last forward through that plan.  Create the plan with exact=True (whole history,
+last forward through that plan: a backward whose forward's history has been replaced
+(another forward_exact, a new model, the autotuner) raises instead of returning the
+gradient of another forward.  Create the plan with exact=True (whole history,
 ns * nt * (nx+2npml) * ld floats) or exact='checkpoint' (states every ~sqrt(nt) steps,
 segments recomputed in the backward; the same gradient bit for bit).
 """
@@ -25,11 +27,15 @@ class _Seismograms(torch.autograd.Function):
         plan.set_model(vp.detach().to(torch.float32), rho)
         seis = plan.forward_exact(src)
         ctx.plan = plan
+        ctx.generation = plan.exact_generation()
         ctx.vp_dtype = vp.dtype
         return seis
 
     @staticmethod
     def backward(ctx, gseis):
+        if ctx.plan.exact_generation() != ctx.generation:
+            raise RuntimeError("the plan's divergence history no longer belongs to this forward (another "
+                               "forward_exact, a new model or the autotuner ran on the plan since)")
         grad = ctx.plan.adjoint_exact(gseis.contiguous())
         return grad.to(ctx.vp_dtype), None, None, None
 

@@ -123,19 +123,6 @@ struct StripRef {
     int iz, ix;
 };
 
-// Receiver of a tile: receiver index and cell.
-struct RecRef {
-    int idx;
-    int iz, ix;
-    int pad;
-};
-
-// Injection group: the receivers of one tile that share one cell, in receiver order.
-struct InjGroup {
-    int e0, e1;     // entry range in the shot's injection list
-};
-
-
 // ---- 16-bit boundary strips ---------------------------------------------------------
 // Strip mode SM: 0 FP32 (the reference's values), 1 FP16, 2 BF16.  In the 16-bit modes
 // the values of one shot and step are scaled by 2^k, the power of two that puts the
@@ -1118,6 +1105,8 @@ struct BatchedState {
     float *R[2] = {nullptr, nullptr};   // reconstruction state [ns][3][nx][nz]
     float *end = nullptr;        // saved final forward state [ns][3][nx][nz]
     float *src = nullptr;        // plan-owned copy of the source traces
+    float *src_ex = nullptr;     // the exact forward's own copy (its checkpointed sweep reruns it)
+    int64_t exact_gen = 0;       // count of exact forwards; names the history dhist holds
     float *acc = nullptr;        // [ns][3][nx][nz] per-shot FP32 accumulators
     float *dhist = nullptr;      // [ns][nt or ck][nxp][ld] divergence history (exact adjoint)
     int ck = 0;                  // checkpoint interval (0: the whole history is kept)

@@ -100,4 +100,17 @@ __device__ __forceinline__ float p_undo(float p, float alpha,
     return p + alpha * (kC1 * (u0 - um1 + w0 - wm1) + kC2 * (up1 - um2 + wp1 - wm2));
 }
 
+// A receiver's index and cell (1-based padded row, column).
+struct RecRef {
+    int idx;
+    int iz, ix;
+    int pad;
+};
+
+// Injection group: the receivers (of one tile, or of the whole grid) that share one
+// cell, in receiver order.
+struct InjGroup {
+    int e0, e1;     // entry range in the shot's injection list
+};
+
 }  // namespace swit

@@ -49,14 +49,20 @@ __device__ __forceinline__ float at(const Dims &d, const float *f, int iz, int i
     (void)k
 
 // lam: [ns][3][np]; q: [ns][2][np]; gk: [ns][np]; D: divergence history (DSave)
-__global__ void k_ex_inject(Dims d, int it, const ShotGeom *__restrict__ geo, const int *__restrict__ rec_cells,
-                            const float *__restrict__ g, float *__restrict__ lam) {
+// One thread per recording cell (the shot's cell groups), adding its receivers'
+// dJ/dseis in receiver order, as the fused step does.
+__global__ void k_ex_inject(Dims d, int it, const int *__restrict__ ngrp, const InjGroup *__restrict__ grp,
+                            const RecRef *__restrict__ lst, const float *__restrict__ g, float *__restrict__ lam) {
     const int s = blockIdx.y;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= geo[s].ng) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ngrp[s]) return;
     const size_t np = (size_t)d.nxp * d.ld;
-    atomicAdd(&lam[(size_t)s * 3 * np + rec_cells[(size_t)s * d.ngmax + r]],
-              g[((size_t)s * d.ngmax + r) * d.nt + (it - 1)]);
+    const InjGroup q = grp[(size_t)s * d.ngmax + t];
+    const RecRef *l = lst + (size_t)s * d.ngmax;
+    float *c = lam + (size_t)s * 3 * np + pidx(d, l[q.e0].iz, l[q.e0].ix);
+    float acc = *c;
+    for (int e = q.e0; e < q.e1; ++e) acc += g[((size_t)s * d.ngmax + l[e].idx) * d.nt + (it - 1)];
+    *c = acc;
 }
 
 // grid (ceil(nxp/256), ns): one thread per column

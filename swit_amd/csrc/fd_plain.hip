@@ -24,20 +24,34 @@ __global__ void k_setup_coef(Dims d, const float *__restrict__ vp, const float *
 }
 
 // vmin over interior rows 2..nz (math.f90:342-361 via solver.f90:73), then the
-// damping profile (pml.f90:80-87).  One workgroup.
-__global__ void k_setup_pml(Dims d, const float *__restrict__ vp, float *__restrict__ prof,
-                            float *__restrict__ vmin_out) {
-    __shared__ float red[1024];
-    float v = 10000000000.0f;
+// damping profile (pml.f90:80-87).  One workgroup.  Also the model statistics the
+// plan checks before any step runs: stats[0] = max vp (the stability limit of
+// toolbox/base.py:57-59), stats[1] = number of vp or rho values that are not finite
+// and positive.
+__global__ void k_setup_pml(Dims d, const float *__restrict__ vp, const float *__restrict__ rho,
+                            float *__restrict__ prof, float *__restrict__ vmin_out, float *__restrict__ stats) {
+    __shared__ float red[1024], rmax[1024];
+    __shared__ int rbad[1024];
+    float v = 10000000000.0f, vmax = 0.0f;
+    int bad = 0;
     const int n = d.nx * d.nz;
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
         int iz = k % d.nz + 1;
-        if (iz > 1) v = fminf(v, vp[k]);
+        const float c = vp[k], r = rho[k];
+        if (iz > 1) v = fminf(v, c);
+        vmax = fmaxf(vmax, c);
+        bad += !(c > 0.0f && c <= 3.0e38f) + !(r > 0.0f && r <= 3.0e38f);
     }
     red[threadIdx.x] = v;
+    rmax[threadIdx.x] = vmax;
+    rbad[threadIdx.x] = bad;
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] = fminf(red[threadIdx.x], red[threadIdx.x + s]);
+        if (threadIdx.x < s) {
+            red[threadIdx.x] = fminf(red[threadIdx.x], red[threadIdx.x + s]);
+            rmax[threadIdx.x] = fmaxf(rmax[threadIdx.x], rmax[threadIdx.x + s]);
+            rbad[threadIdx.x] += rbad[threadIdx.x + s];
+        }
         __syncthreads();
     }
     float vmin = red[0];
@@ -50,6 +64,8 @@ __global__ void k_setup_pml(Dims d, const float *__restrict__ vp, float *__restr
     if (threadIdx.x == 0) {
         prof[0] = 0.0f;
         *vmin_out = vmin;
+        stats[0] = rmax[0];
+        stats[1] = (float)rbad[0];
     }
 }
 
@@ -286,17 +302,23 @@ __global__ void k_back_p(Dims d, int shot0, const float *__restrict__ B,
                            w[k], w[k - 1], w[k + 1], w[k - 2]);
 }
 
-// Adjoint-source injection (:348-367).  sadj_t: [ns][nt][ngmax].
-__global__ void k_back_inject(Dims d, int it, int shot0, const ShotGeom *__restrict__ geo,
-                              const int *__restrict__ rec_inj, const float *__restrict__ sadj_t,
-                              float *__restrict__ Pn) {
+// Adjoint-source injection (:348-367).  sadj_t: [ns][nt][ngmax].  One thread per
+// injection cell (cell groups of the shot: ngrp[s] groups, each a range of the shot's
+// receiver list sorted stably by cell), adding its receivers in receiver order as the
+// reference's loop does, so shared cells give the reference's bits every run.
+__global__ void k_back_inject(Dims d, int it, int shot0, const int *__restrict__ ngrp,
+                              const InjGroup *__restrict__ grp, const RecRef *__restrict__ lst,
+                              const float *__restrict__ sadj_t, float *__restrict__ Pn) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int s = shot0 + blockIdx.z;
-    if (t >= geo[s].ng) return;
-    size_t off = (size_t)s * d.nxp * d.ld;
-    int cell = rec_inj[(size_t)s * d.ngmax + t];
-    float v = d.dt * sadj_t[((size_t)s * d.nt + (it - 1)) * d.ngmax + t];
-    atomicAdd(Pn + off + cell, v);
+    if (t >= ngrp[s]) return;
+    const InjGroup g = grp[(size_t)s * d.ngmax + t];
+    const RecRef *l = lst + (size_t)s * d.ngmax;
+    float *c = Pn + (size_t)s * d.nxp * d.ld + pidx(d, l[g.e0].iz, l[g.e0].ix);
+    const float *a = sadj_t + ((size_t)s * d.nt + (it - 1)) * d.ngmax;
+    float acc = *c;
+    for (int e = g.e0; e < g.e1; ++e) acc = acc + d.dt * a[l[e].idx];
+    *c = acc;
 }
 
 // Unconditional p mirror (:370-373) and imaging (solver.f90:237-239):

@@ -63,6 +63,11 @@ struct swit_plan {
     ShotGeom *geo = nullptr;
     int *rec_fwd = nullptr;       // [ns][ngmax] padded offset of the recording cell
     int *rec_inj = nullptr;       // [ns][ngmax] padded offset of the adjoint injection cell
+    // cell groups (receivers sharing a cell, in receiver order) of the injection cells
+    // (inj_*: plain adjoint) and of the recording cells (rec_*: exact adjoint's passes)
+    int *inj_ngrp = nullptr, *rec_ngrp = nullptr;                 // [ns]
+    InjGroup *inj_grp = nullptr, *rec_grp = nullptr;              // [ns][ngmax]
+    RecRef *inj_lst = nullptr, *rec_lst = nullptr;                // [ns][ngmax]
     uint32_t *strip_map = nullptr;
     // model
     float *K = nullptr, *B = nullptr, *damp = nullptr, *prof = nullptr, *vmin = nullptr;
@@ -75,6 +80,7 @@ struct swit_plan {
     float *strips = nullptr;
     float *seis_t = nullptr, *sadj_t = nullptr;
     float *dg = nullptr, *dfw = nullptr, *dbk = nullptr;
+    float *stats_h = nullptr;     // pinned host copy of the model statistics of set_model
     bool model_set = false;
     bool strips_valid = false;
     BatchedState bat;
@@ -98,6 +104,8 @@ struct swit_plan {
     void release() {
         for (void *q : allocs) (void)hipFree(q);
         allocs.clear();
+        if (stats_h) (void)hipHostFree(stats_h);
+        stats_h = nullptr;
     }
 };
 
@@ -108,6 +116,35 @@ struct swit_plan {
             return rc_;           \
         }                         \
     } while (0)
+
+static int build_injection(swit_plan *pl, const std::vector<int> &cells, const TileMap &tm, int margin, int cap,
+                           std::vector<int> &gst, std::vector<InjGroup> &groups, std::vector<RecRef> &list);
+
+// Cell groups over the whole padded grid (one tile): per shot, the group count, the
+// groups and the receiver list they index, uploaded to the plan's device arrays.
+static int upload_cell_groups(swit_plan *pl, const std::vector<int> &cells, int **ngrp, InjGroup **grp,
+                              RecRef **lst) {
+    const Dims &d = pl->d;
+    const int ns = pl->ns;
+    TileMap all{};
+    all.ztop = 1;
+    all.th = d.nzp;
+    all.tx = d.nxp;
+    all.nzt = all.nxt = all.ntiles = all.cg = 1;
+    std::vector<int> gst;
+    std::vector<InjGroup> groups;
+    std::vector<RecRef> list;
+    TRY(build_injection(pl, cells, all, 0, d.ngmax, gst, groups, list));
+    std::vector<int> cnt(ns);
+    for (int s = 0; s < ns; ++s) cnt[s] = gst[(size_t)s * 2 + 1];
+    TRY(pl->alloc(ngrp, ns));
+    TRY(pl->alloc(grp, groups.size()));
+    TRY(pl->alloc(lst, list.size()));
+    HIP_TRY(hipMemcpy(*ngrp, cnt.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(*grp, groups.data(), sizeof(InjGroup) * groups.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(*lst, list.data(), sizeof(RecRef) * list.size(), hipMemcpyHostToDevice));
+    return SWIT_OK;
+}
 
 static int plan_build(swit_plan *pl, const swit_grid *g, const swit_shots *sh, int flags) {
     if (!g || !sh) return set_err(SWIT_EINVAL, "grid and shots are required");
@@ -124,6 +161,8 @@ static int plan_build(swit_plan *pl, const swit_grid *g, const swit_shots *sh, i
     if (s16 && !batched) return set_err(SWIT_EINVAL, "16-bit strips need the batched path");
     if ((flags & (SWIT_PLAN_EXACT | SWIT_PLAN_EXACT_CKPT)) && !batched)
         return set_err(SWIT_EINVAL, "the exact adjoint needs the batched path");
+    const bool nostrips = (flags & SWIT_PLAN_NO_STRIPS) != 0;
+    if (nostrips && s16) return set_err(SWIT_EINVAL, "a plan without strips takes no strip precision");
 
     Dims &d = pl->d;
     d.nx = g->nx;
@@ -209,6 +248,7 @@ static int plan_build(swit_plan *pl, const swit_grid *g, const swit_shots *sh, i
     TRY(pl->alloc(&pl->damp, np));
     TRY(pl->alloc(&pl->prof, 1024));
     TRY(pl->alloc(&pl->vmin, 64));
+    HIP_TRY(hipHostMalloc((void **)&pl->stats_h, 4 * sizeof(float), hipHostMallocDefault));
     TRY(pl->alloc(&pl->c_int, ni));
     TRY(pl->alloc(&pl->rho_int, ni));
     if (!batched) {
@@ -230,7 +270,7 @@ static int plan_build(swit_plan *pl, const swit_grid *g, const swit_shots *sh, i
         TRY(pl->alloc(&pl->dfw, ni * ns));
         TRY(pl->alloc(&pl->dbk, ni * ns));
     }
-    if (!s16) TRY(pl->alloc(&pl->strips, (size_t)ns * d.nt * d.S));
+    if (!s16 && !nostrips) TRY(pl->alloc(&pl->strips, (size_t)ns * d.nt * d.S));
     TRY(pl->alloc(&pl->seis_t, (size_t)ns * d.nt * ngmax));
     TRY(pl->alloc(&pl->sadj_t, (size_t)ns * d.nt * ngmax));
     // Receiver columns past a shot's own count are never written by a kernel; they
@@ -242,6 +282,9 @@ static int plan_build(swit_plan *pl, const swit_grid *g, const swit_shots *sh, i
     HIP_TRY(hipMemcpy(pl->rec_fwd, rf.data(), sizeof(int) * rf.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(pl->rec_inj, ri.data(), sizeof(int) * ri.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(pl->strip_map, sm.data(), sizeof(uint32_t) * sm.size(), hipMemcpyHostToDevice));
+    if (!batched) TRY(upload_cell_groups(pl, ri, &pl->inj_ngrp, &pl->inj_grp, &pl->inj_lst));
+    if (flags & (SWIT_PLAN_EXACT | SWIT_PLAN_EXACT_CKPT))
+        TRY(upload_cell_groups(pl, rf, &pl->rec_ngrp, &pl->rec_grp, &pl->rec_lst));
     if (batched) TRY(batched_init(pl));
     return SWIT_OK;
 }
@@ -304,9 +347,23 @@ extern "C" int swit_plan_set_model(swit_plan *pl, const float *vp, const float *
     const Dims &d = pl->d;
     dim3 blk(64, 4);
     dim3 grd(cdiv(d.nzp, 64), cdiv(d.nxp, 4), 1);
-    hipLaunchKernelGGL(k_setup_coef, grd, blk, 0, stream, d, vp, rho, pl->K, pl->B);
+    // the model's statistics first; a refused model leaves the plan without a model
+    pl->model_set = false;
+    pl->strips_valid = false;
+    pl->bat.exact_valid = false;
+    hipLaunchKernelGGL(k_setup_pml, dim3(1), dim3(1024), 0, stream, d, vp, rho, pl->prof, pl->vmin, pl->vmin + 1);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_setup_pml, dim3(1), dim3(1024), 0, stream, d, vp, pl->prof, pl->vmin);
+    HIP_TRY(hipMemcpyAsync(pl->stats_h, pl->vmin + 1, 2 * sizeof(float), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (pl->stats_h[1] != 0.0f)
+        return set_err(SWIT_EINVAL, "the model holds %.0f vp or rho values that are not finite and positive",
+                       (double)pl->stats_h[1]);
+    // toolbox/base.py:57-59: 4th-order stability needs dt < sqrt(3/8) dx / max(vp)
+    const double dt_req = std::sqrt(3.0 / 8.0) * (double)d.dx / (double)pl->stats_h[0];
+    if (dt_req <= (double)d.dt)
+        return set_err(SWIT_EINVAL, "modeling stability: dt = %.4f ms > dt_required = %.4f ms (max vp %.1f m/s)",
+                       (double)d.dt * 1000.0, dt_req * 1000.0, (double)pl->stats_h[0]);
+    hipLaunchKernelGGL(k_setup_coef, grd, blk, 0, stream, d, vp, rho, pl->K, pl->B);
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_setup_damp, grd, blk, 0, stream, d, pl->prof, pl->damp);
     LAUNCH_CHECK();
@@ -376,7 +433,7 @@ static int plain_adjoint(swit_plan *pl, const float *src, hipStream_t st) {
             hipLaunchKernelGGL(k_back_p, gp, blk, 0, st, d, s, pl->B, pl->damp, pl->Pb[cur], pl->Pb[1 - cur],
                                pl->Ub, pl->Wb);
             hipLaunchKernelGGL(k_back_inject, dim3(cdiv(d.ngmax, 256), 1, 1), dim3(256), 0, st, d, it, s,
-                               pl->geo, pl->rec_inj, pl->sadj_t, pl->Pb[1 - cur]);
+                               pl->inj_ngrp, pl->inj_grp, pl->inj_lst, pl->sadj_t, pl->Pb[1 - cur]);
             hipLaunchKernelGGL(k_back_mirror_image, gp, blk, 0, st, d, s, pl->Pb[cur], pl->Pb[1 - cur],
                                pl->Pr, pl->dg, pl->dfw, pl->dbk);
             LAUNCH_CHECK();
@@ -395,6 +452,8 @@ extern "C" int swit_forward(swit_plan *pl, const float *src, float *seis, int sa
     if (!pl->model_set) return set_err(SWIT_ESTATE, "swit_plan_set_model must be called first");
     const Dims &d = pl->d;
     if (save_strips < 0 || save_strips > 2) return set_err(SWIT_EINVAL, "save_strips must be 0, 1 or 2");
+    if (save_strips == 1 && (pl->flags & SWIT_PLAN_NO_STRIPS))
+        return set_err(SWIT_EINVAL, "the plan was created without strips (SWIT_PLAN_NO_STRIPS)");
     const int save = save_strips == 1;
     if (save_strips != 2) pl->strips_valid = false;
     if (pl->flags & SWIT_PATH_BATCHED)
@@ -696,9 +755,10 @@ static int batched_init(swit_plan *pl) {
         HIP_TRY(hipMemcpy(bs.exinj_groups, eg.data(), sizeof(InjGroup) * eg.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(bs.exinj_list, el.data(), sizeof(RecRef) * el.size(), hipMemcpyHostToDevice));
         TRY(pl->alloc(&bs.gseis, (size_t)ns * ngmax * d.nt));
+        TRY(pl->alloc(&bs.src_ex, (size_t)ns * d.nt));
     }
     bs.strip_mode = (pl->flags & SWIT_STRIPS_F16) ? 1 : (pl->flags & SWIT_STRIPS_BF16) ? 2 : 0;
-    if (bs.strip_mode) {
+    if (bs.strip_mode && !(pl->flags & SWIT_PLAN_NO_STRIPS)) {
         TRY(pl->alloc(&bs.strips16, (size_t)ns * d.nt * d.S));
         TRY(pl->alloc(&bs.smax, (size_t)ns * d.nt));
         TRY(pl->alloc(&bs.ring, (size_t)kStripRing * ns * d.S));
@@ -801,14 +861,14 @@ static int forward_prologue(swit_plan *pl, int save, hipStream_t st) {
 // Forward step it.  With strip saving the step's FP32 strip row goes to the plan's
 // strips (FP32 plans) or to the ring that k_strip_encode empties (16-bit plans).
 static void launch_fwd(swit_plan *pl, int it, const float *in, float *out, int save, hipStream_t st,
-                       DSave dsave = DSave{nullptr, 0, 0}) {
+                       DSave dsave = DSave{nullptr, 0, 0}, const float *src = nullptr) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
     float *strips = !save ? nullptr : bs.strip_mode == 0 ? pl->strips : bs.ring;
     const int s_rows = bs.strip_mode == 0 ? d.nt : kStripRing;
     const int s_row = bs.strip_mode == 0 ? it - 1 : (it - 1) % kStripRing;
     hipLaunchKernelGGL(k_fwd_fused, dim3(bs.ngroups_f * bs.fwd.ntiles), dim3(kThreads), 0, st, d, it, bs.fwd,
-                       pl->ns, bs.Gf, bs.CP, bs.CV, pl->geo, bs.src, in, out, bs.rec_start, bs.rec_list,
+                       pl->ns, bs.Gf, bs.CP, bs.CV, pl->geo, src ? src : bs.src, in, out, bs.rec_start, bs.rec_list,
                        pl->seis_t, bs.strip_start, bs.strip_list, strips, s_rows, s_row, dsave);
 }
 
@@ -1032,8 +1092,9 @@ static int time_steps(swit_plan *pl, int which, int steps, hipStream_t st, doubl
     auto launch = [&](int k) {
         if (which == 0) {
             const int it = 2 + k % (d.nt - 1);
-            // with strip saving, as the gradient's forward runs
-            launch_fwd(pl, it, bs.F[k & 1], bs.F[(k + 1) & 1], 1, st);
+            // with strip saving, as the gradient's forward runs (a plan without strips
+            // only runs forwards that save none)
+            launch_fwd(pl, it, bs.F[k & 1], bs.F[(k + 1) & 1], (pl->flags & SWIT_PLAN_NO_STRIPS) ? 0 : 1, st);
         } else {
             // the plan's backward kernel; a two-step launch starts at it >= 3
             const int it = spl == 2 ? d.nt - (2 * k) % (d.nt - 2) : d.nt - k % (d.nt - 1);
@@ -1092,7 +1153,9 @@ extern "C" int swit_plan_autotune(swit_plan *pl, int steps, int *choice, double 
     }
     int bsteps = bs.bwd_steps, bwi = bs.bwd_wi;
     const int wi0 = bs.bwd_wi;
-    for (int sp = 1; sp <= 2; ++sp) {
+    // the backward kernels read strips: a plan without them has no backward to tune
+    const int sp_max = (pl->flags & SWIT_PLAN_NO_STRIPS) ? 0 : 2;
+    for (int sp = 1; sp <= sp_max; ++sp) {
         if (sp == 2 && pl->d.nt < 3) continue;
         bs.bwd_steps = sp;
         // the one-step kernel's tile width is a candidate too; the two-step kernel keeps
@@ -1126,7 +1189,7 @@ extern "C" int swit_plan_autotune(swit_plan *pl, int steps, int *choice, double 
     choice[3] = bsteps;
     choice[4] = bs.bwd.tx;
     us[0] = best_f;
-    us[1] = best_b;
+    us[1] = sp_max ? best_b : 0.0;
     return SWIT_OK;
 }
 
@@ -1328,6 +1391,8 @@ extern "C" int swit_group_forward(swit_group *gr, const float *const *src, float
     for (size_t i = 0; i < n; ++i) {
         if (!src[i] || !seis[i]) return set_err(SWIT_EINVAL, "plan %zu: src and seis are required", i);
         if (!gr->plans[i]->model_set) return set_err(SWIT_ESTATE, "plan %zu: no model set", i);
+        if (save && (gr->plans[i]->flags & SWIT_PLAN_NO_STRIPS))
+            return set_err(SWIT_EINVAL, "plan %zu was created without strips (SWIT_PLAN_NO_STRIPS)", i);
     }
     for (size_t i = 0; i < n; ++i) {
         swit_plan *pl = gr->plans[i];
@@ -1502,6 +1567,8 @@ extern "C" int swit_group_gradient(swit_group *gr, int kind, const float *const 
             !illum_back[i])
             return set_err(SWIT_EINVAL, "plan %zu: missing array", i);
         if (!pl->model_set) return set_err(SWIT_ESTATE, "plan %zu: no model set", i);
+        if (pl->flags & SWIT_PLAN_NO_STRIPS)
+            return set_err(SWIT_EINVAL, "plan %zu was created without strips (SWIT_PLAN_NO_STRIPS)", i);
         TRY(misfit_check(kind, pl->ns * pl->d.ngmax, pl->d.nt));
         for (const void *q : {(const void *)obs[i], (const void *)seis[i], (const void *)adj[i],
                               (const void *)work[i], (const void *)misfit[i], (const void *)grad[i],
@@ -1558,7 +1625,7 @@ static int exact_forward_steps(swit_plan *pl, int a, int b, DSave ds, hipStream_
     for (int it = a + 1; it <= b; ++it) {
         const float *in = bs.F[it % 2];
         float *out = bs.F[(it + 1) % 2];
-        launch_fwd(pl, it, in, out, 0, st, ds);
+        launch_fwd(pl, it, in, out, 0, st, ds, bs.src_ex);
         LAUNCH_CHECK();
     }
     return SWIT_OK;
@@ -1600,7 +1667,8 @@ static int exact_adjoint_steps(swit_plan *pl, int a, int b, DSave ds, int &k, hi
     const dim3 blk(64, 4), grd(cdiv(d.nzp, 64), cdiv(d.nxp, 4), ns);
     const dim3 gcol(cdiv(d.nxp, 256), ns), ginj(cdiv(d.ngmax, 256), ns);
     for (int it = b; it >= a + 1; --it) {
-        hipLaunchKernelGGL(ex::k_ex_inject, ginj, dim3(256), 0, st, d, it, pl->geo, pl->rec_fwd, bs.gseis, bs.lam);
+        hipLaunchKernelGGL(ex::k_ex_inject, ginj, dim3(256), 0, st, d, it, pl->rec_ngrp, pl->rec_grp, pl->rec_lst,
+                           bs.gseis, bs.lam);
         if (d.free_surface == 1) hipLaunchKernelGGL(ex::k_ex_mirror, gcol, dim3(256), 0, st, d, 2, bs.lam);
         hipLaunchKernelGGL(ex::k_ex_vel, grd, blk, 0, st, d, bs.KB, bs.CV, bs.lam, bs.qbuf);
         hipLaunchKernelGGL(ex::k_ex_p, grd, blk, 0, st, d, bs.qbuf, bs.lam);
@@ -1655,7 +1723,7 @@ extern "C" int swit_forward_exact(swit_plan *pl, const float *src, float *seis, 
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
     bs.exact_valid = false;
-    HIP_TRY(hipMemcpyAsync(bs.src, src, sizeof(float) * pl->ns * d.nt, hipMemcpyDeviceToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(bs.src_ex, src, sizeof(float) * pl->ns * d.nt, hipMemcpyDeviceToDevice, stream));
     TRY(forward_prologue(pl, 0, stream));
     if (bs.use_graph) {
         if (!bs.gex_f) TRY(exact_capture(pl, 0, &bs.gex_f));
@@ -1667,7 +1735,12 @@ extern "C" int swit_forward_exact(swit_plan *pl, const float *src, float *seis, 
                        d.ngmax, pl->seis_t, seis);
     LAUNCH_CHECK();
     bs.exact_valid = true;
+    ++bs.exact_gen;
     return SWIT_OK;
+}
+
+extern "C" int64_t swit_plan_exact_generation(const swit_plan *pl) {
+    return pl && pl->bat.exact_valid ? pl->bat.exact_gen : 0;
 }
 
 extern "C" int swit_adjoint_exact(swit_plan *pl, const float *gseis, double *grad, hipStream_t stream) {

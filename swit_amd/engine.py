@@ -42,7 +42,9 @@ class Plan:
     """One solver plan: grid, acquisition and device scratch for every shot.
 
     path: 'plain' (per-shot, per-step launches) or 'batched' (all shots per launch).
-    strips: 'f32' | 'f16' | 'bf16' storage of the boundary strips.
+    strips: 'f32' | 'f16' | 'bf16' storage of the boundary strips, or 'none' for a plan
+    that only models (observed data, line-search forwards, the exact adjoint): it holds
+    no strips, and strip-saving forwards and adjoint() are refused.
     """
 
     def __init__(self, nx, nz, dx, dt, nt, npml, src_xz, rec_xz, free_surface=1,
@@ -58,7 +60,8 @@ class Plan:
         self.nx, self.nz, self.nt, self.npml = int(nx), int(nz), int(nt), int(npml)
         self.dx, self.dt = float(np.float32(dx)), float(np.float32(dt))
         flags = {"plain": _native.SWIT_PATH_PLAIN, "batched": _native.SWIT_PATH_BATCHED}[path]
-        flags |= {"f32": 0, "f16": _native.SWIT_STRIPS_F16, "bf16": _native.SWIT_STRIPS_BF16}[strips]
+        flags |= {"f32": 0, "f16": _native.SWIT_STRIPS_F16, "bf16": _native.SWIT_STRIPS_BF16,
+                  "none": _native.SWIT_PLAN_NO_STRIPS}[strips]
         if exact:
             # exact=True keeps the whole divergence history; exact='checkpoint' keeps
             # states every ~sqrt(nt) steps and recomputes segments in the adjoint
@@ -91,9 +94,27 @@ class Plan:
             raise ValueError(f"expected shape {tuple(shape)}, got {tuple(t.shape)}")
         return t
 
+    def _out(self, out):
+        """A seismogram output: a new (ns, ngmax, nt) float32 tensor, or `out` checked to
+        be exactly that (contiguous, on the plan's device), since the kernels write
+        ns * ngmax * nt floats through its pointer."""
+        torch = self._torch
+        shape = (self.ns, self.ngmax, self.nt)
+        if out is None:
+            return torch.empty(shape, dtype=torch.float32, device=self.device)
+        if (not torch.is_tensor(out) or out.dtype != torch.float32 or tuple(out.shape) != shape
+                or out.device != self.device or not out.is_contiguous()):
+            raise ValueError(f"out must be a contiguous float32 tensor of shape {shape} on {self.device}")
+        return out
+
     @property
     def scratch_bytes(self):
         return int(self._lib.swit_plan_scratch_bytes(self._h))
+
+    def exact_generation(self):
+        """Generation of the divergence history the plan holds (raised by every
+        forward_exact; 0 when none is valid)."""
+        return int(self._lib.swit_plan_exact_generation(self._h))
 
     def set_option(self, option, value):
         """option: 'graph' | 'profile' | 'fwd_group' | 'fwd_colgroup' | 'bwd_colgroup' |
@@ -145,8 +166,7 @@ class Plan:
         torch = self._torch
         save_strips = int(save_strips)
         src = self._dev(src, torch.float32, (self.ns, self.nt))
-        if out is None:
-            out = torch.empty((self.ns, self.ngmax, self.nt), dtype=torch.float32, device=self.device)
+        out = self._out(out)
         _native.check(self._lib.swit_forward(self._h, ctypes.c_void_p(src.data_ptr()),
                                              ctypes.c_void_p(out.data_ptr()), save_strips,
                                              self._stream()))
@@ -170,8 +190,7 @@ class Plan:
         history for adjoint_exact().  Needs a plan created with exact=True."""
         torch = self._torch
         src = self._dev(src, torch.float32, (self.ns, self.nt))
-        if out is None:
-            out = torch.empty((self.ns, self.ngmax, self.nt), dtype=torch.float32, device=self.device)
+        out = self._out(out)
         _native.check(self._lib.swit_forward_exact(self._h, ctypes.c_void_p(src.data_ptr()),
                                                    ctypes.c_void_p(out.data_ptr()), self._stream()))
         self._src = src

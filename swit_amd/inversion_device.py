@@ -1,26 +1,34 @@
-"""The inversion loop of the reference toolbox (toolbox/inversion.py:30-482) with every
-vector resident on the device: model, gradient, search direction, the L-BFGS S / Y
-history and the seismograms stay in HBM, and one iteration runs forward (boundary
-strips kept on the device), processing (swit_process), misfit and adjoint source
-(swit_misfit), adjoint and gradient preconditioning (swit_grad_precond) without SU
-or .bin files in between.
+"""FWI on one GPU with every vector resident in HBM: the inversion loop of the
+reference toolbox (toolbox/inversion.py:30-504) as its callers run it, without SU or
+.bin files in between.  The model, gradient, search direction, the L-BFGS history and
+the seismograms stay on the device; one iteration runs the forward with boundary
+strips kept on the device, processing (swit_process), misfit and adjoint source
+(swit_misfit), the adjoint and gradient preconditioning (swit_grad_precond).
 
-The loop keeps the file-based loop's call order and decisions (swit_amd.inversion,
-itself inversion.py's): forward with saved strips, processing, misfit, adjoint,
-NLCG (Polak-Ribiere, restart after 6, descent check) or L-BFGS (memory 5, Liu-Nocedal
-scaling, descent check, restart on failure) with float32 S / Y, backtracking with
-forward-only runs and the same step safeguards and clipping, then the decision.  It
-writes the same outputs under homepath/outputs (gradient / direction / velocity per
-iteration, misfit and line-search logs).  Inner products accumulate in FP64 on the
-device; the reference takes float32 ones of its float32 history, so directions agree
-to float32 rounding.
+The decisions are the reference loop's, so that the misfit history and the models
+match it:
+  * NLCG, Polak-Ribiere+: steepest descent on the first step, after 6 steps, on
+    beta <= 0 and on a non-descent direction (inversion.py:111-171);
+  * L-BFGS with 5 pairs, newest first, float32 history, Liu-Nocedal scaling; steepest
+    descent, and a cleared history, on the first step or a non-descent direction
+    (inversion.py:174-271);
+  * backtracking (inversion.py:274-338): at most 6 trials from optim.step_length (the
+    first L-BFGS step and every NLCG step) or from a step that bounds the largest
+    model change to [20, 300] m/s, halving; trial models clipped to [vpmin, vpmax];
+  * accept or restart (inversion.py:403-449), and the same outputs under
+    homepath/outputs (tools.py:340-359): gradient, direction and model per iteration,
+    the misfit and line-search logs, and the reference's log lines.
+Inner products accumulate in FP64 on the device, where the reference forms float32
+ones of its float32 history, so the directions agree to float32 rounding.
 """
 from __future__ import annotations
 
 import numpy as np
 
-from . import inversion as _host
+import os
+
 from . import preprocess, su
+from .tools import savebinfloat32
 
 
 def _torch():
@@ -127,209 +135,188 @@ class DeviceProblem:
         self.lanes.close()
 
 
-# -- search directions (inversion.py:112-263) -----------------------------------------------
-def NLCG(inv_scheme):
-    """inversion.py:112-171 on device vectors."""
-    cg_itmax = 6
-    g_now, g_old, d_old = inv_scheme['g_now'], inv_scheme['g_old'], inv_scheme['d_old']
-    cg_it = inv_scheme['cg_it'] + 1
-    if cg_it == 1:
-        d_now = -g_now
-        status = 0
-    elif cg_it > cg_itmax:
-        d_now = -g_now
-        status = 1
-        print('restarting NLCG... [periodic restart]')
-    else:
-        beta = _dot(g_now, g_now - g_old) / _dot(g_old, g_old)
-        if beta <= 0.:
-            beta = 0.
-            status = 1
-            print('restarting NLCG... [negative beta]')
-        else:
-            status = 0
-        d_now = -g_now + beta * d_old
-    if _dot(d_now, g_now) / _dot(g_now, g_now) > 0.:
-        d_now = -g_now
-        status = 1
-        print('restarting NLCG... [not a descent direction]')
-    else:
-        print('find NLCG direction')
-    if status:
-        cg_it = 0
-    inv_scheme['d_now'] = d_now
-    inv_scheme['cg_it'] = cg_it
-    return inv_scheme
+# -- the loop ---------------------------------------------------------------------------
+_MEMORY = 5                 # L-BFGS pairs
+_CG_RESTART = 6             # NLCG steps before a periodic restart
+_TRIALS = 6                 # line-search trials
+_DV_MAX, _DV_MIN = 300.0, 20.0   # bounds on the largest model change of a later L-BFGS step
 
 
-def lbfgs_direction(S, Y, k, g_now):
-    """Two-loop recursion over the k newest pairs (row 0 newest) with the Liu-Nocedal
-    initial scaling (inversion.py:222-239); S, Y: (memory, n) float32 arrays.  The
-    updates scale the float32 rows in FP64, as numpy does for a float64 scalar times a
-    float32 column."""
-    f64 = _torch().float64
-    rho = [0.0] * k
-    alpha = [0.0] * k
-    q = g_now.to(f64)
-    for i in range(k):
-        rho[i] = 1. / _dot(Y[i], S[i])
-        alpha[i] = rho[i] * _dot(S[i], q)
-        q = q - alpha[i] * Y[i].to(f64)
-    z = (_dot(Y[0], S[0]) / _dot(Y[0], Y[0])) * q
-    for i in range(k - 1, -1, -1):
-        beta = rho[i] * _dot(Y[i], z)
-        z = z + (alpha[i] - beta) * S[i].to(f64)
-    return z
-
-
-def LBFGS(inv_scheme):
-    """inversion.py:174-263 on device vectors; the float32 history rows shift down by
-    one (row 0 newest) as the reference's memmap columns do."""
-    memory = inv_scheme['memory']
-    memory_used = inv_scheme['memory_used']
-    lbfgs_it = inv_scheme['lbfgs_it'] + 1
-    g_now, g_old = inv_scheme['g_now'], inv_scheme['g_old']
-    v_now, v_old = inv_scheme['v_now'], inv_scheme['v_old']
-    S, Y = inv_scheme['S'], inv_scheme['Y']
-    if lbfgs_it == 1:
-        d_now = -g_now
-        status = 0
-    else:
-        s = v_now - v_old
-        y = g_now - g_old
-        if memory_used > 0:
-            S[1:] = S[:-1].clone()
-            Y[1:] = Y[:-1].clone()
-        S[0] = s
-        Y[0] = y
-        memory_used = min(memory_used + 1, memory)
-        z = lbfgs_direction(S, Y, memory_used, g_now)
-        if _dot(-z, g_now) / _dot(g_now, g_now) > 0:
-            d_now = -g_now
-            status = 1
-            print('restarting LBFGS... [not a descent direction]')
-        else:
-            d_now = -z
-            status = 0
-    if status:
-        lbfgs_it = 1
-        memory_used = 0
-        S.zero_()
-        Y.zero_()
-    else:
-        print('find LBFGS direction')
-    inv_scheme['memory_used'] = memory_used
-    inv_scheme['lbfgs_it'] = lbfgs_it
-    inv_scheme['d_now'] = d_now
-    return inv_scheme
-
-
-# -- line search, decisions, state (inversion.py:266-474) ------------------------------------
 def model_update(vp, update, vpmax, vpmin):
-    """inversion.py:465-474: step, then the reference's two masked assignments in its
-    order (values above vpmax become vpmax, then values below vpmin become vpmin)."""
+    """vp + update bounded as the reference bounds it (inversion.py:465-474): entries
+    above vpmax become vpmax, then entries below vpmin become vpmin."""
     torch = _torch()
     v = vp + update
     v = torch.where(v > vpmax, torch.full_like(v, vpmax), v)
     return torch.where(v < vpmin, torch.full_like(v, vpmin), v)
 
 
-def backtrack(problem, optim, inv_scheme):
-    """inversion.py:266-338 (see swit_amd.inversion.backtrack), with forward-only misfit
-    evaluations of the device problem."""
-    dec, search_max, vmax_thresh, vmin_thresh = 0.5, 6, 300, 20
-    f_now, v_now, d_now = inv_scheme['f_now'], inv_scheme['v_now'], inv_scheme['d_now']
-    if optim.scheme in ['LBFGS']:
-        dmax = float(d_now.abs().max())
-        if inv_scheme['lbfgs_it'] == 1:
+def lbfgs_direction(S, Y, k, g):
+    """H g by the two-loop recursion over the k newest pairs (row 0 newest) of the
+    float32 histories S, Y (memory, n), with H0 = (y0.s0 / y0.y0) I; the float32 rows
+    are scaled in FP64."""
+    f64 = _torch().float64
+    rho = [1.0 / _dot(Y[i], S[i]) for i in range(k)]
+    alpha = [0.0] * k
+    q = g.to(f64)
+    for i in range(k):
+        alpha[i] = rho[i] * _dot(S[i], q)
+        q = q - alpha[i] * Y[i].to(f64)
+    z = (_dot(Y[0], S[0]) / _dot(Y[0], Y[0])) * q
+    for i in reversed(range(k)):
+        z = z + (alpha[i] - rho[i] * _dot(Y[i], z)) * S[i].to(f64)
+    return z
+
+
+class _State:
+    """The loop's state on the device: current and previous model, gradient and
+    direction, misfits, the method's counters and the L-BFGS history."""
+
+    def __init__(self, v0, scheme):
+        torch = _torch()
+        self.scheme = scheme
+        self.v = v0.to(torch.float64)
+        n, dev = self.v.numel(), self.v.device
+        self.v_prev = self.v.clone()
+        self.v_try = self.v
+        self.f = self.f_prev = self.f_try = 0.0
+        self.g = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.d = self.g
+        self.step, self.trials, self.failed = 0.0, 0, False
+        self.S = self.Y = None
+        self.restart()
+
+    def restart(self):
+        """Forget the method's memory (a new start of NLCG / L-BFGS)."""
+        torch = _torch()
+        n, dev = self.v.numel(), self.v.device
+        self.g_prev = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.d_prev = self.g_prev
+        self.count = 0              # NLCG steps since a restart / L-BFGS iteration number
+        self.used = 0               # L-BFGS pairs held
+        if self.scheme == 'LBFGS':
+            self.S = torch.zeros((_MEMORY, n), dtype=torch.float32, device=dev)
+            self.Y = torch.zeros((_MEMORY, n), dtype=torch.float32, device=dev)
+
+    def direction(self):
+        if self.scheme == 'NLCG':
+            self._nlcg()
+        else:
+            self._lbfgs()
+
+    def _nlcg(self):
+        g = self.g
+        self.count += 1
+        restart = False
+        if self.count == 1:
+            d = -g
+        elif self.count > _CG_RESTART:
+            d, restart = -g, True
+            print('restarting NLCG... [periodic restart]')
+        else:
+            beta = _dot(g, g - self.g_prev) / _dot(self.g_prev, self.g_prev)
+            if beta <= 0.0:
+                beta, restart = 0.0, True
+                print('restarting NLCG... [negative beta]')
+            d = -g + beta * self.d_prev
+        if _dot(d, g) / _dot(g, g) > 0.0:
+            d, restart = -g, True
+            print('restarting NLCG... [not a descent direction]')
+        else:
+            print('find NLCG direction')
+        if restart:
+            self.count = 0
+        self.d = d
+
+    def _lbfgs(self):
+        g = self.g
+        self.count += 1
+        if self.count == 1:
+            self.d = -g
+            print('find LBFGS direction')
+            return
+        if self.used:
+            self.S[1:] = self.S[:-1].clone()
+            self.Y[1:] = self.Y[:-1].clone()
+        self.S[0] = self.v - self.v_prev
+        self.Y[0] = g - self.g_prev
+        self.used = min(self.used + 1, _MEMORY)
+        z = lbfgs_direction(self.S, self.Y, self.used, g)
+        if _dot(-z, g) / _dot(g, g) > 0:
+            self.d = -g
+            self.count, self.used = 1, 0
+            self.S.zero_()
+            self.Y.zero_()
+            print('restarting LBFGS... [not a descent direction]')
+        else:
+            self.d = -z
+            print('find LBFGS direction')
+
+    def line_search(self, problem, optim):
+        if self.scheme == 'LBFGS' and self.count != 1:
+            dmax = float(self.d.abs().max())
+            step = _DV_MAX / dmax if dmax > _DV_MAX else _DV_MIN / dmax if dmax < _DV_MIN else 1.0
+        else:
             step = optim.step_length
-        elif dmax > vmax_thresh:
-            step = 1.0 * vmax_thresh / dmax
-        elif dmax < vmin_thresh:
-            step = 1.0 * vmin_thresh / dmax
-        else:
-            step = 1.0
-    else:
-        step = optim.step_length
-    ls_iter = 0
-    while True:
-        ls_iter += 1
-        v_try = model_update(v_now, d_now * step, optim.vpmax, optim.vpmin)
-        f_try = problem.misfit(v_try)
-        if f_try >= f_now:
-            print("line search: step = %.4f, f_now = %.4e, f_try = %.4e. Not decrease." % (step, f_now, f_try))
-            width = dec
-        else:
-            inv_scheme = _host.updata_inv_scheme(inv_scheme, 0, ls_iter, step, f_try, v_try)
-            print("line search: step = %.4f, f_now = %.4e, f_try = %.4e. Succeed." % (step, f_now, f_try))
-            return inv_scheme
-        if ls_iter >= search_max:
-            inv_scheme = _host.updata_inv_scheme(inv_scheme, -1, ls_iter, step, f_try, v_try)
-            print("line search fails: exceed the iteration of linesearch.")
-            return inv_scheme
-        step = step * width
+        for trial in range(1, _TRIALS + 1):
+            v_try = model_update(self.v, self.d * step, optim.vpmax, optim.vpmin)
+            f_try = problem.misfit(v_try)
+            self.trials, self.step, self.f_try, self.v_try = trial, step, f_try, v_try
+            if f_try < self.f:
+                self.failed = False
+                print("line search: step = %.4f, f_now = %.4e, f_try = %.4e. Succeed." % (step, self.f, f_try))
+                return
+            print("line search: step = %.4f, f_now = %.4e, f_try = %.4e. Not decrease." % (step, self.f, f_try))
+            step *= 0.5
+        self.failed = True
+        print("line search fails: exceed the iteration of linesearch.")
 
-
-def optimize_init(n, optim, device):
-    """inversion.py:341-400 on the device (memory 5 for L-BFGS)."""
-    torch = _torch()
-    temp = torch.zeros(n, dtype=torch.float64, device=device)
-    common = {'g_now': temp, 'g_old': temp, 'd_now': temp, 'v_old': temp, 'v_now': temp,
-              'v_try': temp, 'f_old': 0., 'f_now': 0., 'f_try': 0., 'ls_iter': 0, 'ls_status': 0,
-              'ls_step': 0.}
-    if optim.scheme in ['NLCG']:
-        return dict(common, d_old=temp, cg_it=0)
-    memory = 5
-    return dict(common, memory=memory, memory_used=0, lbfgs_it=0,
-                S=torch.zeros((memory, n), dtype=torch.float32, device=device),
-                Y=torch.zeros((memory, n), dtype=torch.float32, device=device))
-
-
-def desicion_make(n, optim, inv_scheme, device):
-    """inversion.py:403-449."""
-    if optim.scheme in ['NLCG']:
-        if inv_scheme['ls_status'] == -1 and inv_scheme['cg_it'] > 1:
-            inv_scheme['cg_it'] = 0
-            inv_scheme['g_old'] = inv_scheme['g_old'] * 0.
-            inv_scheme['d_old'] = inv_scheme['d_old'] * 0.
+    def decide(self):
+        """Take the trial model, or keep the model and restart the method."""
+        if self.scheme == 'NLCG' and self.failed and self.count > 1:
+            self.count = 0
+            self.g_prev = self.g_prev * 0.0
+            self.d_prev = self.d_prev * 0.0
+            print('line search fails: restarting the line search ...')
+        elif self.scheme == 'LBFGS' and self.failed:
+            self.restart()
+            self.v_prev = self.v
+            self.f_prev = 0.0
             print('line search fails: restarting the line search ...')
         else:
-            inv_scheme['g_old'] = inv_scheme['g_now']
-            inv_scheme['d_old'] = inv_scheme['d_now']
-            inv_scheme['f_old'] = inv_scheme['f_now']
-            inv_scheme['f_now'] = inv_scheme['f_try']
-            inv_scheme['v_now'] = inv_scheme['v_try']
+            self.g_prev, self.d_prev, self.v_prev = self.g, self.d, self.v
+            self.f_prev, self.f = self.f, self.f_try
+            self.v = self.v_try
             print('accept line search result')
-    elif optim.scheme in ['LBFGS']:
-        if inv_scheme['ls_status'] == -1:
-            old = inv_scheme
-            inv_scheme = optimize_init(n, optim, device)
-            for key in ('v_now', 'f_now', 'g_now', 'd_now'):
-                inv_scheme[key] = old[key]
-            inv_scheme['v_old'] = old['v_now']
-            print('line search fails: restarting the line search ...')
-        else:
-            inv_scheme['g_old'] = inv_scheme['g_now']
-            inv_scheme['v_old'] = inv_scheme['v_now']
-            inv_scheme['v_now'] = inv_scheme['v_try']
-            inv_scheme['f_old'] = inv_scheme['f_now']
-            inv_scheme['f_now'] = inv_scheme['f_try']
-            print('accept line search result')
-    return inv_scheme
 
 
-def save_inv_scheme(simu, optim, inv_scheme):
-    """toolbox/tools.py:340-359 from device vectors (copied to the host once each)."""
-    host = dict(inv_scheme)
-    for key in ('g_now', 'd_now', 'v_now'):
-        host[key] = inv_scheme[key].cpu().numpy()
-    _host.save_inv_scheme(simu, optim, host)
+def _log(path, value, integer=False):
+    """One value per line appended to a text log, '%d' or '%.4e' (tools.py:307-319)."""
+    with open(path, 'a') as f:
+        f.write(('%d\n' if integer else '%.4e\n') % value)
+
+
+def save_outputs(simu, optim, st):
+    """Per-iteration outputs under homepath/outputs, as tools.py:340-359 writes them:
+    gradient, direction and model as float32 .bin files; the misfit log (the starting
+    misfit first, at the first iteration), the line-search step and trial count."""
+    out = simu.system.homepath + 'outputs'
+    it = optim.iter
+    for sub, name, vec in (('gradient', 'grad', st.g), ('direction', 'dirc', st.d), ('velocity', 'vp', st.v)):
+        os.makedirs(out + '/' + sub, exist_ok=True)
+        savebinfloat32(out + '/%s/%s-%d.bin' % (sub, name, it), vec.cpu().numpy())
+    if it == 1:
+        _log(out + '/misfit_data.dat', st.f_prev)
+    _log(out + '/misfit_data.dat', st.f)
+    _log(out + '/line_search_step.dat', st.step)
+    _log(out + '/line_search_iteration.dat', st.trials, integer=True)
 
 
 def inversion(simu, optim, inv_model, problem=None, obs=None, simu_true=None):
-    """inversion.py:30-79 with device-resident vectors.  problem: a DeviceProblem (one is
-    built from simu, optim and obs / simu_true when None).  Returns the final state
-    (device vectors); simu.model.vp holds the final model on the host."""
+    """The loop of inversion.py:30-79 with device-resident vectors.  problem: a
+    DeviceProblem (or any object with misfit(v) and gradient(v) -> (f, g) on device
+    vectors); one is built from simu, optim and obs / simu_true when None.  Returns the
+    final state; simu.model.vp holds the final model on the host."""
     torch = _torch()
     simu.model.vp = inv_model['vp']
     simu.model.rho = inv_model['rho']
@@ -337,27 +324,21 @@ def inversion(simu, optim, inv_model, problem=None, obs=None, simu_true=None):
     if own:
         problem = DeviceProblem(simu, optim, obs=obs, simu_true=simu_true)
     nx, nz = problem.nx, problem.nz
-    n = nx * nz
     dev = problem.device
-    inv_scheme = optimize_init(n, optim, dev)
     v0 = torch.from_numpy(np.ascontiguousarray(np.asarray(simu.model.vp, dtype=np.float32)).reshape(-1)).to(dev)
-    inv_scheme['v_now'] = v0.to(torch.float64)
-    inv_scheme['v_old'] = v0.to(torch.float64)
+    st = _State(v0, optim.scheme)
     try:
         while optim.iter < optim.maxiter:
             optim.iter += 1
             print('\n-----------  iteration %d  -----------\n' % optim.iter)
-            inv_scheme['f_now'], inv_scheme['g_now'] = problem.gradient(inv_scheme['v_now'])
-            if optim.scheme in ['NLCG']:
-                inv_scheme = NLCG(inv_scheme)
-            elif optim.scheme in ['LBFGS']:
-                inv_scheme = LBFGS(inv_scheme)
-            inv_scheme = backtrack(problem, optim, inv_scheme)
-            inv_scheme = desicion_make(n, optim, inv_scheme, dev)
-            save_inv_scheme(simu, optim, inv_scheme)
+            st.f, st.g = problem.gradient(st.v)
+            st.direction()
+            st.line_search(problem, optim)
+            st.decide()
+            save_outputs(simu, optim, st)
         print('\n-----------  iteration end  -----------\n')
     finally:
-        simu.model.vp = inv_scheme['v_now'].cpu().numpy().reshape(nx, nz)
+        simu.model.vp = st.v.cpu().numpy().reshape(nx, nz)
         if own:
             problem.close()
-    return inv_scheme
+    return st

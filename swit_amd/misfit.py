@@ -143,14 +143,22 @@ def misfit(simu, misfit_type):
 
 
 def adjoint_source(simu, misfit_type):
-    """Adjoint sources of every shot, array (srcn, recn, nt) (misfit.py:74-88)."""
+    """Adjoint sources of every shot, float32 array (srcn, recn, nt) (misfit.py:74-88).
+    float32 is the precision the solver reads them in (the reference writes them to
+    src%d.bin as float32, toolbox/solver.py:112-113); each shot's result is stored as it
+    is computed, so the host holds one gather set and one shot's temporaries (at
+    801x267x6000 with 81 shots: 1.6 GB, where a list of float64 gathers stacked into
+    one array would hold 6.2 GB at its peak)."""
     kind = misfit_type.lower()
-    out = []
+    out = None
     for isrc in range(simu.source.n):
         if kind == 'rtm':
             obs, _, _ = _gathers(simu.system.homepath, isrc, need_syn=False)
-            out.append(obs.astype(np.float64))
-            continue
-        obs, syn, dt = _gathers(simu.system.homepath, isrc)
-        out.append(_ADJ[kind](obs, syn, dt))
-    return np.array(out)
+            a = obs
+        else:
+            obs, syn, dt = _gathers(simu.system.homepath, isrc)
+            a = _ADJ[kind](obs, syn, dt)
+        if out is None:
+            out = np.empty((simu.source.n,) + a.shape, np.float32)
+        out[isrc] = a
+    return out

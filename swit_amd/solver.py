@@ -15,10 +15,12 @@ example workflows):
 Instead of `mpirun fd2dmpi` with files in between, both jobs run on the GPU through
 libswit_fd.so (swit_amd.engine.Plan, batched path): the boundary strips and final
 state of the last strip-saving 'syn' forward stay in device memory for the adjoint.
-Observed-data forwards run on a transient plan of their own and never disturb that
-state.  Inputs are rounded exactly as the reference's file round trip rounds them (float32
-models and sources, '%f' dx/dt, '%10.1f' coordinates).  Any solver failure raises
-ValueError('Forward solver crash' / 'Adjoint solver crash') as the reference does.
+Observed-data forwards never disturb that state: they run on a plan of their own that
+holds no strips and is kept for the next call.  Seismograms go to the host and into
+their SU files one shot at a time.  Inputs are rounded exactly as the reference's file
+round trip rounds them (float32 models and sources, '%f' dx/dt, '%10.1f' coordinates).
+Any solver failure raises ValueError('Forward solver crash' / 'Adjoint solver crash')
+as the reference does.
 """
 from __future__ import annotations
 
@@ -37,14 +39,12 @@ WRITE_PARFILES = False
 PATH = "batched"
 # Boundary-strip storage: 'f32' (reference precision), 'f16' or 'bf16'.
 STRIPS = "f32"
-# grad_precond of adjoint(): 'device' (GPU kernels, equal to the numpy steps to FP64
-# rounding) or 'host' (the numpy restatement of toolbox/postprocess.py).
-PRECOND = "device"
 # Time the batched step kernels' launch configurations once per synthetic-data plan,
 # at its first model, and keep the fastest (the results do not depend on the choice).
 AUTOTUNE = True
 
-_JOBS = {}
+_JOBS = {}         # synthetic-data plan per homepath
+_OBS_JOBS = {}     # observed-data plan (no strips) per homepath
 
 
 def _misfit_module():
@@ -60,12 +60,12 @@ def _postprocess_module():
 class _Job:
     """Device plan and saved-state bookkeeping for one homepath and geometry."""
 
-    def __init__(self, key, inputs):
+    def __init__(self, key, inputs, strips=None):
         from .engine import Plan
         self.key = key
         self.plan = Plan(inputs["nx"], inputs["nz"], inputs["dx"], inputs["dt"], inputs["nt"],
                          inputs["npml"], inputs["src_xz"], inputs["rec_xz"],
-                         free_surface=inputs["free_surface"], path=PATH, strips=STRIPS)
+                         free_surface=inputs["free_surface"], path=PATH, strips=strips or STRIPS)
         self.fwd_src = None          # source traces of the last strip-saving forward
         self.model = None            # (vp, rho) the plan currently holds
         self.tuned = None            # the autotuner's choice, once made
@@ -119,12 +119,26 @@ def _job(simu, inputs):
     return job
 
 
-def release(homepath=None):
-    """Free the device plans (all, or the one of homepath)."""
-    for h in ([homepath] if homepath else list(_JOBS)):
-        job = _JOBS.pop(h, None)
+def _obs_job(simu, inputs):
+    """The observed-data plan of simu's homepath: no strips, kept across calls (created
+    or rebuilt on a new geometry)."""
+    key = ("obs",) + _key(inputs)
+    home = simu.system.homepath
+    job = _OBS_JOBS.get(home)
+    if job is None or job.key != key:
         if job is not None:
             job.plan.close()
+        job = _OBS_JOBS[home] = _Job(key, inputs, strips="none")
+    return job
+
+
+def release(homepath=None):
+    """Free the device plans (all, or those of homepath)."""
+    for jobs in (_JOBS, _OBS_JOBS):
+        for h in ([homepath] if homepath else list(jobs)):
+            job = jobs.pop(h, None)
+            if job is not None:
+                job.plan.close()
 
 
 def forward(simu, simu_type='obs', savesnap=0):
@@ -152,21 +166,21 @@ def forward(simu, simu_type='obs', savesnap=0):
             # adjoint(); a run without snapshots drops them, as cleandata drops the files
             job = _job(simu, inp)
             job.set_model(inp["vp"], inp["rho"], tune=AUTOTUNE)
-            seis = job.plan.forward(torch.from_numpy(src32), save_strips=1 if savesnap == 1 else 0).cpu().numpy()
+            seis = job.plan.forward(torch.from_numpy(src32), save_strips=1 if savesnap == 1 else 0)
             job.fwd_src = src32 if savesnap == 1 else None
         else:
-            # observed data run on a plan of their own, so the synthetic plan's model
-            # and saved state (data/syn/ in the reference) are left as they are
-            obs = _Job(_key(inp), inp)
-            try:
-                obs.set_model(inp["vp"], inp["rho"])
-                seis = obs.plan.forward(torch.from_numpy(src32), save_strips=0).cpu().numpy()
-            finally:
-                obs.plan.close()
+            # observed data run on a plan of their own, holding no strips, so the
+            # synthetic plan's model and saved state (data/syn/ in the reference) are
+            # left as they are
+            obs = _obs_job(simu, inp)
+            obs.set_model(inp["vp"], inp["rho"])
+            seis = obs.plan.forward(torch.from_numpy(src32), save_strips=0)
+        # one shot's gather at a time to the host and into its SU file
         ng = simu.receiver.n
         for isrc in range(srcn):
-            su.write(homepath + 'data/%s/src%d_sg.su' % (simu_type, isrc + 1), seis[isrc, :ng],
+            su.write(homepath + 'data/%s/src%d_sg.su' % (simu_type, isrc + 1), seis[isrc, :ng].cpu().numpy(),
                      inp["dt"], inp["src_xz"][isrc, 0], inp["rec_xz"][isrc, :, 0])
+        del seis
     except Exception as exc:  # the reference reports a failed solver run this way
         print(exc)
         raise ValueError('Forward solver crash') from exc
@@ -206,8 +220,6 @@ def adjoint(simu, optim):
     grad = loadbinfloat32(homepath + 'data/syn/src0_kernel_vp.bin')
     forw = loadbinfloat32(homepath + 'data/syn/src0_illum_forw.bin')
     back = loadbinfloat32(homepath + 'data/syn/src0_illum_back.bin')
-    if PRECOND == 'device':
-        return _postprocess_module().grad_precond_device(simu, optim, grad, forw, back)
     return _postprocess_module().grad_precond(simu, optim, grad, forw, back)
 
 

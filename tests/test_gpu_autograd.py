@@ -171,13 +171,13 @@ def test_checkpointed_exact_adjoint_equals_full_history(nt):
 ])
 def test_fused_exact_step_equals_separate_passes(nx, nz, npml, nt, ns, mode, fs):
     """The fused exact-adjoint step kernel (one launch per reverse step) gives the seven
-    separate passes' gradient bit for bit (every receiver has a cell of its own), with
-    the full history and with checkpoints, with and without the free surface, for
-    tilings that put the mirror rows at many positions in the tiles' windows."""
+    separate passes' gradient bit for bit, with the full history and with checkpoints,
+    with and without the free surface, for tilings that put the mirror rows at many
+    positions in the tiles' windows.  Both add the receivers that share a recording cell
+    (the case's last receiver shares a surface receiver's) in receiver order."""
     import torch
     from swit_amd.engine import Plan
     w = workloads.small(seed=nx + nz + npml, nx=nx, nz=nz, npml=npml, nt=nt, ns=ns)
-    w.rec_xz = w.rec_xz[:, :w.nx + 1]      # the last receiver shares a surface receiver's cell
     src = torch.from_numpy(integrated_sources(w))
     out = {}
     for fused in (1, 0):

@@ -30,7 +30,8 @@ def _optim():
 
 @pytest.mark.gpu
 def test_dropin_sequence_matches_oracle(tmp_path):
-    from swit_amd import misfit, postprocess, preprocess, solver, su, tools
+    from oracle import precond as P
+    from swit_amd import misfit, preprocess, solver, su, tools
     w = workloads.small(seed=21, nx=56, nz=36, npml=10, nt=300, ns=3)
     w.rec_xz = w.rec_xz[:, :w.nx]                     # surface receivers, all inside x range
     home = str(tmp_path) + "/"
@@ -62,11 +63,10 @@ def test_dropin_sequence_matches_oracle(tmp_path):
     np.testing.assert_array_equal(kern, a["grad"].astype(np.float32).ravel())
     np.testing.assert_array_equal(tools.loadbinfloat32(home + 'data/syn/src0_illum_forw.bin'),
                                   a["illum_forw"].astype(np.float32).ravel())
-    ref = postprocess.grad_precond(simu, optim, a["grad"].astype(np.float32).ravel(),
-                                   a["illum_forw"].astype(np.float32).ravel(),
-                                   a["illum_back"].astype(np.float32).ravel())
-    # adjoint() preconditions on the device (solver.PRECOND = 'device'): the numpy steps to
-    # FP64 rounding
+    f32 = [a[k].astype(np.float32) for k in ("grad", "illum_forw", "illum_back")]
+    ref = P.precondition(*f32, optim.vpmax, True, optim.grad_mute, optim.grad_thred, optim.grad_smooth,
+                         optim.grad_mask).ravel()
+    # adjoint() preconditions on the device: the oracle's numpy steps to FP64 rounding
     assert np.linalg.norm(g - ref) <= 1e-12 * np.linalg.norm(ref)
     r = f["seis"].astype(np.float64) - obs
     assert J == pytest.approx(sum(misfit.misfit_waveform(obs[i], f["seis"][i], inp["dt"]) for i in range(w.ns)))
@@ -96,12 +96,35 @@ def test_obs_forward_keeps_saved_state_and_syn_without_snapshots_clears_it(tmp_p
     solver.release()
 
 
+def _file_loop_callbacks(simu, optim):
+    """The reference loop's solver calls through files (toolbox/inversion.py:48-60 for a
+    gradient, :319-325 for a line-search trial) as callbacks of the oracle loop."""
+    from swit_amd import misfit, preprocess, solver
+    nx, nz = simu.model.nx, simu.model.nz
+
+    def evaluate(v):
+        simu.model.vp = np.asarray(v, np.float64).reshape(nx, nz)
+        solver.forward(simu, 'syn', savesnap=1)
+        preprocess.process_workflow(simu, optim, 'syn')
+        f = misfit.misfit(simu, optim.misfit_type)
+        return f, solver.adjoint(simu, optim)
+
+    def trial(v):
+        simu.model.vp = np.asarray(v, np.float64).reshape(nx, nz)
+        solver.forward(simu, 'syn', savesnap=0)
+        preprocess.process_workflow(simu, optim, 'syn')
+        return misfit.misfit(simu, optim.misfit_type)
+    return evaluate, trial
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("scheme", ["LBFGS", "NLCG"])
-def test_inversion_loop_reduces_misfit(tmp_path, scheme):
-    """Three iterations of the reference inversion loop (swit_amd.inversion) on the
-    drop-in solver: every accepted model lowers the waveform misfit."""
-    from swit_amd import base, inversion, preprocess, solver
+def test_reference_loop_on_the_dropin_reduces_misfit(tmp_path, scheme):
+    """Three iterations of the reference inversion loop's call sequence (the oracle's
+    restatement) on the drop-in solver through its SU and .bin files: every accepted
+    model lowers the waveform misfit."""
+    from oracle import inversion_loop as L
+    from swit_amd import base, preprocess, solver
     w = workloads.small(seed=23, nx=60, nz=36, npml=10, nt=300, ns=3)
     w.rec_xz = w.rec_xz[:, :w.nx]
     home = str(tmp_path) + "/"
@@ -111,11 +134,13 @@ def test_inversion_loop_reduces_misfit(tmp_path, scheme):
     solver.forward(simu_true, 'obs', savesnap=0)
     preprocess.process_workflow(simu_true, optim, 'obs')
     simu = _simu(home, w, w.vp_init)
-    inversion.inversion(simu, optim, {'vp': w.vp_init.astype(np.float64),
-                                      'rho': workloads.gardner(w.vp_init).astype(np.float64)})
-    f = np.loadtxt(home + 'outputs/misfit_data.dat')
+    loop = L.ReferenceLoop(w.vp_init.astype(np.float64).ravel(), scheme, optim.step_length, optim.vpmax, optim.vpmin)
+    evaluate, trial = _file_loop_callbacks(simu, optim)
+    for _ in range(3):
+        loop.iterate(evaluate, trial)
+    f = [loop.f_first] + [h[0] for h in loop.history]
     print(scheme, f)
-    assert len(f) == 4 and np.all(np.diff(f) < 0)
+    assert np.all(np.diff(f) < 0)
     solver.release()
 
 
@@ -123,13 +148,15 @@ def test_inversion_loop_reduces_misfit(tmp_path, scheme):
 @pytest.mark.parametrize("kind,filt,norm,mute", [("Waveform", "None", "None", False),
                                                  ("Globalcorrelation", "Bandpass", ["L2-Trace"], False),
                                                  ("Envelope", "Bandpass", ["Max-Trace"], True)])
-def test_device_loop_matches_file_loop(tmp_path, kind, filt, norm, mute):
+def test_device_loop_matches_reference_loop(tmp_path, kind, filt, norm, mute):
     """The device-resident loop (swit_amd.inversion_device: device processing, misfit,
     adjoint source, preconditioning and L-BFGS vectors, no SU files) against the
-    file-based loop on the drop-in solver over three L-BFGS iterations: the same
-    line-search decisions, misfits to 1e-4 (the logs keep 5 digits) and models to
-    float32 rounding of the different inner products and processing."""
-    from swit_amd import base, inversion, inversion_device, preprocess, solver
+    reference loop's call sequence (the oracle's restatement) on the drop-in solver
+    through its files, over three L-BFGS iterations: the same line-search decisions,
+    misfits to 1e-4 (the logs keep 5 digits) and models to float32 rounding of the
+    different inner products and processing."""
+    from oracle import inversion_loop as L
+    from swit_amd import base, inversion_device, preprocess, solver
     from swit_amd.tools import loadbinfloat32
     w = workloads.small(seed=24, nx=60, nz=36, npml=10, nt=300, ns=3)
     w.rec_xz = w.rec_xz[:, :w.nx]
@@ -145,7 +172,12 @@ def test_device_loop_matches_file_loop(tmp_path, kind, filt, norm, mute):
     o = optim()
     solver.forward(simu_true, 'obs', savesnap=0)
     preprocess.process_workflow(simu_true, o, 'obs')
-    inversion.inversion(_simu(home_a, w, w.vp_init), o, dict(init))
+    simu = _simu(home_a, w, w.vp_init)
+    simu.model.rho = init['rho']
+    ref = L.ReferenceLoop(init['vp'].ravel(), 'LBFGS', o.step_length, o.vpmax, o.vpmin)
+    evaluate, trial = _file_loop_callbacks(simu, o)
+    for _ in range(3):
+        ref.iterate(evaluate, trial)
     solver.release()
 
     home_b = str(tmp_path / 'b') + "/"
@@ -153,14 +185,13 @@ def test_device_loop_matches_file_loop(tmp_path, kind, filt, norm, mute):
     simu = _simu(home_b, w, w.vp_init)
     o = optim()
     inversion_device.inversion(simu, o, dict(init), simu_true=_simu(home_b, w, w.vp_true))
-    fa = np.loadtxt(home_a + 'outputs/misfit_data.dat')
+    fa = np.array([ref.f_first] + [h[0] for h in ref.history])
     fb = np.loadtxt(home_b + 'outputs/misfit_data.dat')
     print(kind, fa, fb)
-    np.testing.assert_array_equal(np.loadtxt(home_a + 'outputs/line_search_iteration.dat'),
-                                  np.loadtxt(home_b + 'outputs/line_search_iteration.dat'))
+    np.testing.assert_array_equal(np.loadtxt(home_b + 'outputs/line_search_iteration.dat'),
+                                  [h[1] for h in ref.history])
     np.testing.assert_allclose(fb, fa, rtol=1e-4)
-    va = loadbinfloat32(home_a + 'outputs/velocity/vp-3.bin')
     vb = loadbinfloat32(home_b + 'outputs/velocity/vp-3.bin')
-    np.testing.assert_allclose(vb, va, rtol=1e-4)
+    np.testing.assert_allclose(vb, ref.models[-1], rtol=1e-4)
     # a failed line search (an L-BFGS restart) keeps the misfit; none increases it
     assert np.all(np.diff(fb) <= 0) and fb[-1] < fb[0]

@@ -1,6 +1,8 @@
 """GPU parity: the HIP path (through the C ABI) against the CPU oracle on the same
-seeded inputs.  FP32 tolerance on relative L2 is 1e-5 (seismograms, gradient,
-illuminations); the error against the FP64 oracle is reported."""
+seeded inputs, both paths: seismograms bit-exact, per-shot FP32 kernels in the
+reference's order (so the FP64 shot sums agree to relative L2 1e-12), also where two
+receivers share an injection cell (the case includes such a pair); the error against
+the FP64 oracle is reported."""
 import numpy as np
 import pytest
 
@@ -8,7 +10,7 @@ from helpers import integrated_sources, rel_l2
 from oracle import oracle as O
 from swit_amd import workloads
 
-TOL_F32 = 1e-5
+TOL_F32 = 1e-12
 
 
 def _oracle(w, vp, src, adj, precision="f32"):
@@ -47,7 +49,7 @@ def test_forward_adjoint_parity(path):
           f"max|diff| {np.abs(seis - f32['seis']).max():.3e}; vs f64 {rel_l2(seis, f64['seis']):.3e}")
     print(f"[{path}] grad relL2 vs f32 {rel_l2(g, a32['grad']):.3e} vs f64 {rel_l2(g, a64['grad']):.3e}; "
           f"fw {rel_l2(fw, a32['illum_forw']):.3e} bk {rel_l2(bk, a32['illum_back']):.3e}")
-    assert rel_l2(seis, f32["seis"]) <= TOL_F32
+    np.testing.assert_array_equal(seis, f32["seis"])
     assert rel_l2(g, a32["grad"]) <= TOL_F32
     assert rel_l2(fw, a32["illum_forw"]) <= TOL_F32
     assert rel_l2(bk, a32["illum_back"]) <= TOL_F32

@@ -1,6 +1,7 @@
-"""grad_precond on the GPU (swit_amd.engine.grad_precond, row f3) against the numpy
-restatement of toolbox/postprocess.py:20-111 (swit_amd.postprocess.grad_precond), which
-tests/test_host_io.py pins: marine and land tapers, with and without gradient smoothing
+"""grad_precond on the GPU (swit_amd.engine.grad_precond, and the drop-in's
+swit_amd.postprocess.grad_precond over it) against the oracle's numpy restatement of
+toolbox/postprocess.py:20-111 (oracle/precond.py, pinned by tests/test_oracle_precond.py):
+marine and land tapers, with and without gradient smoothing
 and mask, small and large spans, grids under 40 cells (span min(nx, nz)/2).
 Tolerance: relative L2 1e-12 (FP64 sums in another order).  With the land taper the
 reference rounds grad * taper to float32 (in-place product on the float32 gradient); a
@@ -10,6 +11,7 @@ each such cell)."""
 import numpy as np
 import pytest
 
+from oracle import precond as P
 from swit_amd import postprocess
 
 
@@ -42,9 +44,9 @@ def test_device_grad_precond_matches_numpy(nx, nz, marine, mute, thred, smooth, 
     simu.model.nx, simu.model.nz = nx, nz
     optim.vpmax, optim.marine_or_land = 3500.0, ('Marine' if marine else 'Land')
     optim.grad_mute, optim.grad_thred, optim.grad_smooth, optim.grad_mask = mute, thred, smooth, mask
-    ref = postprocess.grad_precond(simu, optim, grad.ravel(), forw.ravel(), back.ravel())
+    ref = P.precondition(grad, forw, back, 3500.0, marine, mute, thred, smooth, mask).ravel()
     got = grad_precond(grad, forw, back, 3500.0, marine, mute, thred, smooth, mask).cpu().numpy().ravel()
-    via = postprocess.grad_precond_device(simu, optim, grad.ravel(), forw.ravel(), back.ravel())
+    via = postprocess.grad_precond(simu, optim, grad.ravel(), forw.ravel(), back.ravel())
     np.testing.assert_array_equal(via, got)
     err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
     print(f"{nx}x{nz} marine={marine} mute={mute} smooth={smooth}: rel L2 {err:.2e}")

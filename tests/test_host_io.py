@@ -199,6 +199,7 @@ class _RecordingPlan:
     def __init__(self, nx, nz, dx, dt, nt, npml, src_xz, rec_xz, **kw):
         import torch
         self.ns, self.ng, self.nt, self.nx, self.nz = src_xz.shape[0], rec_xz.shape[1], nt, nx, nz
+        self.strips = kw.get("strips", "f32")
         self.id = len([e for e in self.log if e[0] == "new"])
         self.log.append(("new", self.id))
         self.torch = torch
@@ -226,8 +227,11 @@ class _RecordingPlan:
 def test_dropin_plan_bookkeeping(tmp_path, monkeypatch):
     from swit_amd import engine, preprocess, solver, workloads
     from test_gpu_dropin import _optim, _simu
+    import types
     monkeypatch.setattr(engine, "Plan", _RecordingPlan)
-    monkeypatch.setattr(solver, "PRECOND", "host")     # no GPU here for the preconditioner
+    # no GPU here for the preconditioner: the raw kernel stands in for it
+    monkeypatch.setattr(solver, "_postprocess_module", lambda: types.SimpleNamespace(
+        grad_precond=lambda simu, optim, grad, forw, back: np.asarray(grad, np.float64)))
     _RecordingPlan.log = []
     log = _RecordingPlan.log
     w = workloads.small(seed=22, nx=40, nz=30, npml=8, nt=60, ns=2)
@@ -238,12 +242,15 @@ def test_dropin_plan_bookkeeping(tmp_path, monkeypatch):
     optim = _optim()
     solver.forward(simu, 'syn', savesnap=1)
     solver.forward(simu_true, 'obs', savesnap=0)
-    # the observed run used a plan of its own and closed it; the synthetic plan (0)
-    # saw one model, the autotuner (once, before any strip-saving forward) and one
-    # strip-saving forward
+    # the observed run used a plan of its own (kept for the next observed run); the
+    # synthetic plan (0) saw one model, the autotuner (once, before any strip-saving
+    # forward) and one strip-saving forward
     assert [e for e in log if e[1] == 0] == [("new", 0), ("model", 0, float(np.float32(w.vp_init).astype(np.float64).sum())),
                                              ("tune", 0), ("forward", 0, 1)]
-    assert [e[0] for e in log if e[1] == 1] == ["new", "model", "forward", "close"]
+    assert [e[0] for e in log if e[1] == 1] == ["new", "model", "forward"]
+    assert solver._OBS_JOBS[home].plan.strips == "none"
+    solver.forward(simu_true, 'obs', savesnap=0)           # same model: no new plan, no upload
+    assert [e[0] for e in log if e[1] == 1] == ["new", "model", "forward", "forward"]
     preprocess.process_workflow(simu, optim, 'obs')
     preprocess.process_workflow(simu, optim, 'syn')
     solver.adjoint(simu, optim)
@@ -259,4 +266,4 @@ def test_dropin_plan_bookkeeping(tmp_path, monkeypatch):
     with pytest.raises(ValueError, match='Adjoint solver crash'):
         solver.adjoint(simu, optim)
     solver.release()
-    assert log[-1] == ("close", 0)
+    assert sorted(log[-2:]) == [("close", 0), ("close", 1)]
