@@ -307,10 +307,16 @@ int swit_process_work(const swit_process_params *p, size_t *bytes);
  *            it (required with the window mute);
  *   taper:   device double [nt], obspy's Trace.taper(0.05, 'hann') weights
  *            (required with detrend_taper);
+ *   picks:   device int32 [ngathers*ntr] or NULL: the window mute's centre per trace in
+ *            samples, int((mute_time_ms / 1000 + |mute_late_window|) / dt) from the
+ *            traces' SU header first breaks (preprocess.py:134-139); NULL picks the
+ *            first break from the data (the first sample above 0.1 % of the trace
+ *            maximum, preprocess.py:140-142, 203-209) and shifts it by
+ *            ceil(|mute_late_window| / dt);
  *   work:    device memory of swit_process_work bytes.
  * The filter needs nt > 3 * filter_taps (scipy's padding). */
 int swit_process(const swit_process_params *p, const float *offsets, const double *mute_win, const double *taper,
-                 float *data, void *work, size_t work_bytes, hipStream_t stream);
+                 const int32_t *picks, float *data, void *work, size_t work_bytes, hipStream_t stream);
 
 #ifdef __cplusplus
 }

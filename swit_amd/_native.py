@@ -159,7 +159,7 @@ def load(path: str | None = None):
     L.swit_grad_precond.restype = ctypes.c_int
     L.swit_process_work.argtypes = [ctypes.POINTER(SwitProcessParams), ctypes.POINTER(ctypes.c_size_t)]
     L.swit_process_work.restype = ctypes.c_int
-    L.swit_process.argtypes = [ctypes.POINTER(SwitProcessParams), vp, vp, vp, vp, vp, ctypes.c_size_t, vp]
+    L.swit_process.argtypes = [ctypes.POINTER(SwitProcessParams), vp, vp, vp, vp, vp, vp, ctypes.c_size_t, vp]
     L.swit_process.restype = ctypes.c_int
     pp = ctypes.POINTER(vp)
     L.swit_group_create.argtypes = [pp, ctypes.c_int, pp]

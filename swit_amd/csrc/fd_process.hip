@@ -94,8 +94,11 @@ __device__ __forceinline__ double mute_mask(int i, int itmin, int itmax, int nt,
 }
 
 // Offset and first-break mutes, one workgroup per trace.
+// picks: per-trace first-break window centres (samples) given by the caller, or NULL to
+// pick the first break from the data and shift it by the window (pick_shift).
 __global__ __launch_bounds__(kThreads) void k_mute(ProcArgs p, const float *__restrict__ offsets,
-                                                   const double *__restrict__ win, float *__restrict__ data) {
+                                                   const double *__restrict__ win, const int *__restrict__ picks,
+                                                   float *__restrict__ data) {
     __shared__ float shf[kThreads / 64];
     __shared__ int shi[kThreads / 64];
     const int nt = p.nt;
@@ -107,16 +110,21 @@ __global__ __launch_bounds__(kThreads) void k_mute(ProcArgs p, const float *__re
         return;
     }
     if (!p.mute_arrival) return;
-    float m = 0.0f;
-    for (int i = threadIdx.x; i < nt; i += kThreads) m = fmaxf(m, fabsf(x[i]));
-    m = wmaxf(m, shf);
-    const float thr = 0.001f * m;
-    int first = nt;
-    for (int i = threadIdx.x; i < nt; i += kThreads)
-        if (fabsf(x[i]) > thr) { first = i; break; }
-    first = wmini(first, shi);
-    const int pick = first == nt ? 0 : first;                 // argmax of an all-false mask is 0
-    const double pk = (double)pick + p.pick_shift;
+    double pk;
+    if (picks) {
+        pk = (double)picks[blockIdx.x];
+    } else {
+        float m = 0.0f;
+        for (int i = threadIdx.x; i < nt; i += kThreads) m = fmaxf(m, fabsf(x[i]));
+        m = wmaxf(m, shf);
+        const float thr = 0.001f * m;
+        int first = nt;
+        for (int i = threadIdx.x; i < nt; i += kThreads)
+            if (fabsf(x[i]) > thr) { first = i; break; }
+        first = wmini(first, shi);
+        const int pick = first == nt ? 0 : first;             // argmax of an all-false mask is 0
+        pk = (double)pick + p.pick_shift;
+    }
     const int itmin = (int)(pk - (double)kMuteLength / 2.0);
     const int itmax = (int)((double)itmin + (double)kMuteLength);
     for (int i = threadIdx.x; i < nt; i += kThreads) {

@@ -1921,7 +1921,8 @@ extern "C" int swit_process_work(const swit_process_params *p, size_t *bytes) {
 }
 
 extern "C" int swit_process(const swit_process_params *p, const float *offsets, const double *mute_win,
-                            const double *taper, float *data, void *work, size_t work_bytes, hipStream_t stream) {
+                            const double *taper, const int32_t *picks, float *data, void *work, size_t work_bytes,
+                            hipStream_t stream) {
     g_err[0] = 0;
     if (const int rc = process_check(p)) return rc;
     if (!data || !work) return set_err(SWIT_EINVAL, "data and work are required");
@@ -1950,7 +1951,8 @@ extern "C" int swit_process(const swit_process_params *p, const float *offsets, 
     a.norm_event = p->norm_event;
     const int ntraces = p->ngathers * p->ntr;
     if (a.mute_short || a.mute_long || a.mute_arrival) {
-        hipLaunchKernelGGL(pr::k_mute, dim3(ntraces), dim3(pr::kThreads), 0, stream, a, offsets, mute_win, data);
+        hipLaunchKernelGGL(pr::k_mute, dim3(ntraces), dim3(pr::kThreads), 0, stream, a, offsets, mute_win,
+                           p->mute_late_arrival ? picks : nullptr, data);
         LAUNCH_CHECK();
     }
     if (a.taps) {

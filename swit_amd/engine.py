@@ -525,12 +525,14 @@ def process_params(optim, ngathers, ntr, nt, dt):
     return p, win, taper
 
 
-def process_gathers(optim, data, dt, offsets=None, params=None):
+def process_gathers(optim, data, dt, offsets=None, params=None, first_break_ms=None):
     """process_workflow's steps (toolbox/preprocess.py:47-200) on device gathers, in
     place: data float32 device tensor [ngathers][ntr][nt] (or [ntr][nt]); offsets
     [ngathers][ntr] group x - source x (required with an offset mute).  params: the
     (params, mute window, taper) of process_params, to override the derived ones.
-    Returns data."""
+    first_break_ms: [ngathers][ntr] first-break times in ms from the traces' SU headers
+    (the reference's use_first_break_in_su_header) to centre the window mute on, or None
+    to pick the first break from the data.  Returns data."""
     torch = _require_gpu()
     lib = _native.load()
     if data.dtype != torch.float32 or not data.is_cuda or not data.is_contiguous():
@@ -547,6 +549,13 @@ def process_gathers(optim, data, dt, offsets=None, params=None):
         if off.numel() != ng * ntr:
             raise ValueError("offsets must hold one value per trace")
     w = torch.as_tensor(win, dtype=torch.float64, device=dev) if win is not None else None
+    pk = None
+    if first_break_ms is not None and p.mute_late_arrival:
+        from .preprocess import header_picks
+        fb = np.asarray(first_break_ms).reshape(-1)
+        if fb.size != ng * ntr:
+            raise ValueError("first_break_ms must hold one value per trace")
+        pk = torch.as_tensor(header_picks(fb, optim.mute_late_window, dt).astype(np.int32), device=dev)
     tp = torch.as_tensor(taper, dtype=torch.float64, device=dev) if taper is not None else None
     nbytes = ctypes.c_size_t()
     _native.check(lib.swit_process_work(ctypes.byref(p), ctypes.byref(nbytes)))
@@ -555,6 +564,6 @@ def process_gathers(optim, data, dt, offsets=None, params=None):
 
     def ptr(t):
         return ctypes.c_void_p(t.data_ptr()) if t is not None else None
-    _native.check(lib.swit_process(ctypes.byref(p), ptr(off), ptr(w), ptr(tp), ptr(data), ptr(work), nbytes.value,
-                                   stream))
+    _native.check(lib.swit_process(ctypes.byref(p), ptr(off), ptr(w), ptr(tp), ptr(pk), ptr(data), ptr(work),
+                                   nbytes.value, stream))
     return data

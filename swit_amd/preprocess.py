@@ -120,9 +120,18 @@ def brutal_picker(data):
     return np.array([(np.abs(data[i]) > thr[i]).argmax() for i in range(data.shape[0])])
 
 
-def apply_mute(optim, data, dt, offsets):
+def header_picks(first_break_ms, mute_late_window, dt):
+    """Window-mute centres (samples) from the SU headers' first-break times in ms:
+    int((t_ms / 1000 + |mute_late_window|) / dt) per trace (preprocess.py:134-139)."""
+    t = np.asarray(first_break_ms, dtype=np.float64)
+    return np.trunc((t / 1000.0 + abs(mute_late_window)) / dt).astype(np.int64)
+
+
+def apply_mute(optim, data, dt, offsets, first_break_ms=None):
     """Offset mutes, then the first-break window mute (preprocess.py:98-151), in place
-    on a copy of the float32 gather as the reference's in-place trace products."""
+    on a copy of the float32 gather as the reference's in-place trace products.
+    first_break_ms: per-trace first breaks from the SU headers (the reference's
+    use_first_break_in_su_header), or None to pick them from the data."""
     data = data.copy()
     length = MUTE_LENGTH
     nt = data.shape[-1]
@@ -133,7 +142,10 @@ def apply_mute(optim, data, dt, offsets):
         data[off > optim.mute_offset_long_dis] = 0.0
     if optim.mute_late_arrival:
         mutetype = 'late' if optim.mute_late_window > 0.0 else 'early'
-        pick = brutal_picker(data) + np.ceil(abs(optim.mute_late_window) / dt)
+        if first_break_ms is not None:
+            pick = header_picks(first_break_ms, optim.mute_late_window, dt)
+        else:
+            pick = brutal_picker(data) + np.ceil(abs(optim.mute_late_window) / dt)
         for i in range(data.shape[0]):
             itmin = int(pick[i] - length / 2)
             itmax = int(itmin + length)
@@ -174,9 +186,10 @@ def _identity(optim):
 
 
 def process_workflow(simu, optim, simu_type='syn', use_first_break_in_su_header=False):
-    """Process every shot gather of data/<simu_type>/ (preprocess.py:23-44)."""
-    if use_first_break_in_su_header:
-        raise NotImplementedError('first breaks from SU headers are not supported')
+    """Process every shot gather of data/<simu_type>/ (preprocess.py:23-44).  With
+    use_first_break_in_su_header the window mute is centred on each trace's first break
+    from its SU header (mute-start time, ms) instead of a pick from the data.  The
+    processed file keeps the input traces' headers."""
     datapath = simu.system.homepath + 'data/' + simu_type + '/'
     for isrc in range(simu.source.n):
         load = datapath + 'src%d_sg.su' % (isrc + 1)
@@ -186,15 +199,17 @@ def process_workflow(simu, optim, simu_type='syn', use_first_break_in_su_header=
             continue
         data, hdr = su.read(load)
         dt = float(hdr["dt"][0]) * 1e-6
-        data = apply_mute(optim, data.astype(np.float32), dt, hdr["offset"].astype(np.float64))
+        fb = hdr["muts"].astype(np.float64) if use_first_break_in_su_header else None
+        data = apply_mute(optim, data.astype(np.float32), dt, hdr["offset"].astype(np.float64), fb)
         data = apply_filter(optim, data, dt)
         data = apply_normalize(optim, data)
-        su.write(save, data.astype(np.float32), np.float32(hdr["d1"][0]), hdr["sx"][0], hdr["gx"])
+        su.write_with_headers(save, data.astype(np.float32), hdr)
 
 
-def process_gathers_device(optim, data, dt, offsets=None):
+def process_gathers_device(optim, data, dt, offsets=None, first_break_ms=None):
     """apply_mute, apply_filter and apply_normalize on device gathers (float32 device
     tensor [ngathers][ntr][nt] or [ntr][nt], processed in place) through swit_process;
-    offsets: group x - source x per trace."""
+    offsets: group x - source x per trace; first_break_ms: per-trace first breaks from
+    the SU headers, or None to pick them from the data."""
     from . import engine
-    return engine.process_gathers(optim, data, dt, offsets)
+    return engine.process_gathers(optim, data, dt, offsets, first_break_ms=first_break_ms)

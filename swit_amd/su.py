@@ -8,6 +8,9 @@ fields the reference fills and leaves the rest zero (io.f90:687-774 clean_suhead
     sx = nint(xs)   gx = nint(xg)   offset = gx - sx   cdp = cdpt = nint(0.5 (xs + xg))
     ns = nt (int16)   dt = int(dt * 1e6) (int16, float32 product)
     d1 = dt   f1 = 0   d2 = 1   f2 = 1   (float32)
+
+The mute-start time (muts, int16 ms at byte 110) is zero in the solver's files; field
+data carry first breaks there, which processing can use (toolbox/preprocess.py:134-139).
 """
 from __future__ import annotations
 
@@ -20,7 +23,7 @@ _FIELDS = {
     "tracl": (0, "<i4"), "tracr": (4, "<i4"), "fldr": (8, "<i4"), "tracf": (12, "<i4"),
     "ep": (16, "<i4"), "cdp": (20, "<i4"), "cdpt": (24, "<i4"), "trid": (28, "<i2"),
     "offset": (36, "<i4"), "sx": (72, "<i4"), "sy": (76, "<i4"), "gx": (80, "<i4"),
-    "gy": (84, "<i4"), "ns": (114, "<i2"), "dt": (116, "<i2"),
+    "gy": (84, "<i4"), "muts": (110, "<i2"), "mute": (112, "<i2"), "ns": (114, "<i2"), "dt": (116, "<i2"),
     "d1": (180, "<f4"), "f1": (184, "<f4"), "d2": (188, "<f4"), "f2": (192, "<f4"),
 }
 
@@ -61,11 +64,26 @@ def headers(nt, dt, xs, xg):
     return h
 
 
-def write(path, data, dt, xs, xg):
-    """Write a shot gather data[ng, nt] (float32) with reference headers."""
+def write(path, data, dt, xs, xg, first_break_ms=None):
+    """Write a shot gather data[ng, nt] (float32) with reference headers.  first_break_ms:
+    per-trace first-break times in ms for the mute-start field (muts, int16), which the
+    solver leaves zero; processing can centre its window mute on them."""
     data = np.ascontiguousarray(np.asarray(data, dtype="<f4"))
     ng, nt = data.shape
     h = headers(nt, dt, xs, xg)
+    if first_break_ms is not None:
+        h["muts"] = np.asarray(first_break_ms).reshape(ng)
+    write_with_headers(path, data, h)
+
+
+def write_with_headers(path, data, hdr):
+    """Write data[ng, nt] (float32) under the given 240-byte trace headers (as read),
+    with the sample count set to nt: every other field is kept, as processing that
+    rewrites a gather keeps its traces' headers."""
+    data = np.ascontiguousarray(np.asarray(data, dtype="<f4"))
+    ng, nt = data.shape
+    h = np.array(hdr, dtype=_HDR).reshape(ng)
+    h["ns"] = nt
     rec = np.zeros(ng, dtype=np.dtype([("h", "V240"), ("d", "<f4", (nt,))]))
     rec["h"] = h.view("V240")
     rec["d"] = data

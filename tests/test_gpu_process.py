@@ -1,6 +1,7 @@
 """Device shot-gather processing (swit_process) against the host restatement of
 toolbox/preprocess.py (swit_amd.preprocess, the reference's float32 flow):
-- mutes (offset and first-break window, early and late): bit-exact;
+- mutes (offset and first-break window, early and late; first breaks picked from the
+  data or taken from the SU headers): bit-exact;
 - the filter stage alone: bit-exact against scipy.signal.filtfilt of the float32
   gather, stored as float32, for bandpass, lowpass and highpass designs;
 - filter + demean + linear detrend + taper: to 2e-6 of the trace maximum (scipy fits
@@ -67,6 +68,25 @@ def test_mutes_bit_exact(window):
     want = _host(o, d, dt, off)
     assert np.any(got == 0) and np.any(got != 0)
     np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("window", [0.08, -0.05])
+def test_header_first_break_mute_bit_exact(window):
+    """First breaks from the SU headers (per-trace ms): the device centres the window
+    mute on int((t / 1000 + |window|) / dt) as the host restatement does."""
+    import torch
+    from swit_amd import engine
+    dt = 0.002
+    d, off = _gathers(seed=4)
+    fb = np.random.default_rng(7).integers(0, 1500, size=d.shape[:2])
+    o = _optim(mute_late_arrival=1, mute_late_window=window, mute_offset_short=1, mute_offset_short_dis=30.0)
+    x = torch.from_numpy(d.copy()).cuda()
+    engine.process_gathers(o, x, dt, off, first_break_ms=fb)
+    got = x.cpu().numpy()
+    want = np.array([preprocess.apply_mute(o, d[g], dt, off[g], fb[g]) for g in range(d.shape[0])])
+    np.testing.assert_array_equal(got, want)
+    assert not np.array_equal(got, _device(o, d, dt, off))      # the data picker gives other windows
 
 
 @pytest.mark.gpu

@@ -3,6 +3,8 @@ forms: the mute mask's branches, the first-break pick, obspy's Hann taper, zero-
 filtering of sinusoids in and out of the band, the normalisations, and the
 direct-form II transposed recurrence the device filter uses, against scipy's lfilter
 bit for bit."""
+import os
+
 import numpy as np
 import pytest
 
@@ -125,3 +127,44 @@ def test_df2t_order_is_scipys():
         z[K - 2] = xn * b[K - 1] - yn * a[K - 1]
         y[n] = yn
     assert np.array_equal(y, y_ref) and np.array_equal(z, zf_ref)
+
+
+def test_header_first_break_mute_closed_form(tmp_path):
+    """use_first_break_in_su_header (preprocess.py:134-139): the window centre is
+    int((muts / 1000 + |window|) / dt) from each trace's SU mute-start field (int16 ms at
+    byte 110); a late mute keeps the samples before centre - 50, zeroes those from
+    centre + 50 on and tapers between by 1 - sin; the processed file keeps the headers."""
+    import struct
+    import types
+    from swit_amd import base, preprocess, su
+    nt, dt, ntr = 600, 0.002, 4
+    fb = np.array([100, 250, 0, 733])                    # ms
+    home = str(tmp_path) + '/'
+    os.makedirs(home + 'data/obs')
+    data = np.ones((ntr, nt), np.float32)
+    su.write(home + 'data/obs/src1_sg.su', data, np.float32(dt), np.float32(0.0),
+             np.arange(ntr, dtype=np.float32) * 10.0, first_break_ms=fb)
+    raw = open(home + 'data/obs/src1_sg.su', 'rb').read()
+    rec = 240 + 4 * nt
+    assert [struct.unpack_from('<h', raw, i * rec + 110)[0] for i in range(ntr)] == list(fb)
+    o = base.optimize('Waveform', 'LBFGS', 1, 0.05, 6000.0, 1000.0, 'Marine', 0, 0, 'None', 0.0, 0.0, 1, 0.1,
+                      ['None'], 0, 0, 0.0, 0.0)
+    simu = types.SimpleNamespace(system=types.SimpleNamespace(homepath=home), source=types.SimpleNamespace(n=1))
+    preprocess.process_workflow(simu, o, 'obs', use_first_break_in_su_header=True)
+    out, hdr = su.read(home + 'data/obs/src1_sg_proc.su')
+    np.testing.assert_array_equal(hdr["muts"], fb)
+    _, hin = su.read(home + 'data/obs/src1_sg.su')
+    assert hdr.tobytes() == hin.tobytes()
+    dt_hdr = float(hin["dt"][0]) * 1e-6
+    for i in range(ntr):
+        c = int((fb[i] / 1000.0 + 0.1) / dt_hdr)
+        lo = int(c - 50)
+        expect = 1.0 - preprocess.mask(lo, lo + 100, nt, 100)      # mask branches: test_mask_branches
+        np.testing.assert_allclose(out[i], expect.astype(np.float32), rtol=0, atol=1e-7)
+    # int() truncates: (0.25 + 0.1) / 0.002 evaluates to 174.999..., centre 174
+    assert [int((f / 1000.0 + 0.1) / dt_hdr) for f in fb] == [100, 174, 50, 416]
+    np.testing.assert_array_equal(preprocess.header_picks(fb, 0.1, dt_hdr), [100, 174, 50, 416])
+    # the same gather without header picks: the data picker finds sample 0 of a ones trace
+    preprocess.process_workflow(simu, o, 'obs')
+    out2, _ = su.read(home + 'data/obs/src1_sg_proc.su')
+    assert not np.array_equal(out2, out)
