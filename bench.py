@@ -86,6 +86,7 @@ def autograd_leg(w, src, obs, vp_init, rho_init, args):
         g = step()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    fd = fd_self_check(w, pe, src, obs, vp_init, rho_init, g)
     pe.close()
     # algorithmic bytes of one autograd gradient: the exact forward (36 N_pad + the
     # divergence history 4 N_pad + 4 ng per shot and step) and the fused exact adjoint
@@ -101,7 +102,31 @@ def autograd_leg(w, src, obs, vp_init, rho_init, args):
             "history": "full" if mode is True else "checkpoints", "history_gib": round(need / 2**30, 1),
             "algorithmic_bytes_per_gradient": int(per_grad), "achieved_gbs": round(gbs, 1),
             "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
-            "finite": bool(torch.isfinite(g).all())}
+            "finite": bool(torch.isfinite(g).all()), "fd_self_check": fd}
+
+
+def fd_self_check(w, plan, src, obs, vp, rho, g):
+    """<g, dm> of the exact adjoint against central differences of the same GPU forward
+    (FP32 seismograms, FP64 misfit) along a smooth perturbation dm (N(0,1) noise
+    Gaussian-filtered with sigma 6 cells, max 50 m/s), at eps = 1 and 0.5.  A check that
+    the gradient is the derivative of the forward the GPU runs, at the benchmark's size;
+    the FP32 forward limits it to about 1e-4.  The error against FP64 finite
+    differences of the FP64 oracle is a test (two of these shots)."""
+    import torch
+    from scipy.ndimage import gaussian_filter
+    from swit_amd.autograd import l2_misfit, seismograms
+    rng = np.random.default_rng(22)
+    dm = gaussian_filter(rng.standard_normal((w.nx, w.nz)), 6)
+    dm = torch.from_numpy(dm / np.abs(dm).max() * 50.0).to(vp.device)
+    gd = float((g.double() * dm).sum())
+    out = {"dm_max_m_per_s": 50.0, "g_dot_dm": gd}
+    with torch.no_grad():
+        for eps in (1.0, 0.5):
+            jp = float(l2_misfit(seismograms((vp + eps * dm).float(), plan, src, rho), obs))
+            jm = float(l2_misfit(seismograms((vp - eps * dm).float(), plan, src, rho), obs))
+            fdv = (jp - jm) / (2 * eps)
+            out[f"rel_err_eps_{eps:g}"] = abs(gd - fdv) / abs(fdv)
+    return out
 
 
 def lanes_leg(w, src, obs, vp, rho, g_ref, args, lanes=2):
@@ -129,6 +154,75 @@ def lanes_leg(w, src, obs, vp, rho, g_ref, args, lanes=2):
     return {"lanes": lanes, "value": round(w.ns * args.steps / el, 3), "unit": "shot-gradients/s",
             "ms_per_step": round(1000 * el / args.steps, 2), "grad_rel_vs_main": rel,
             "autotuned": [t.get("bwd_steps") for t in tuned] if tuned else None}
+
+
+def plain_leg(w, src, obs, vp_true, rho_true, vp_init, rho_init, batched_value, args):
+    """The plain per-shot GPU path (one thread per cell, one launch per sweep, shot by
+    shot), two ways, outside the timed steps:
+      * `dropin`: one L2 gradient of all shots through the drop-in solver as its callers
+        run it (swit_amd.solver with PATH='plain': forward with snapshots writing the SU
+        files, processing, misfit, adjoint with the adjoint sources read from the files,
+        the .bin outputs and the preconditioner), observed SU files written first;
+      * `resident`: the plain plan's forward and adjoint with inputs already on the
+        device, as the batched step is timed.
+    batched_over_plain compares the batched step with the drop-in gradient."""
+    import shutil
+    import tempfile
+    import torch
+    from swit_amd import base, misfit as host_misfit, preprocess, solver
+    from swit_amd.engine import Plan
+    k = w.ns if args.plain_shots < 0 else min(args.plain_shots, w.ns)
+    pp = Plan(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, w.src_xz[:k], w.rec_xz[:k], path="plain")
+    pp.set_model(vp_init, rho_init)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    syn_k = pp.forward(src[:k], save_strips=True)
+    pp.adjoint(src[:k], syn_k - obs[:k])
+    torch.cuda.synchronize()
+    resident = time.perf_counter() - t1
+    pp.close()
+    del syn_k
+    # the drop-in, files on a RAM-backed folder when there is one
+    root = "/dev/shm" if os.path.isdir("/dev/shm") else None
+    home = tempfile.mkdtemp(prefix="swit_plain_", dir=root) + "/"
+    wav = np.broadcast_to(workloads_ricker(w), (k, w.nt)).copy()
+
+    def simulate(vp, rho):
+        model = base.model(w.nx, w.nz, w.dx, w.dt, w.nt, True, w.npml, vp.astype(np.float64),
+                           rho.astype(np.float64))
+        return base.simulate(model, base.source(w.f0, k, w.src_xz[:k].astype(np.float64), wav),
+                             base.receiver(w.ng, w.rec_xz[:k].astype(np.float64)), base.system(home, 1))
+    optim = base.optimize('Waveform', 'LBFGS', 1, 0.05, 6000.0, 1000.0, 'Marine', 10, 0, 'None', 1.0, 10.0,
+                          False, 0.0, 'None', False, False, 0.0, 1e9)
+    path0 = solver.PATH
+    solver.PATH = "plain"
+    try:
+        st = simulate(w.vp_true, w.rho(w.vp_true))
+        solver.forward(st, 'obs', savesnap=0)
+        preprocess.process_workflow(st, optim, 'obs')
+        simu = simulate(w.vp_init, w.rho(w.vp_init))
+        t1 = time.perf_counter()
+        solver.forward(simu, 'syn', savesnap=1)
+        preprocess.process_workflow(simu, optim, 'syn')
+        J = host_misfit.misfit(simu, 'Waveform')
+        g = solver.adjoint(simu, optim)
+        dropin = time.perf_counter() - t1
+    finally:
+        solver.PATH = path0
+        solver.release()
+        shutil.rmtree(home, ignore_errors=True)
+    return {"value": round(k / dropin, 3), "unit": "shot-gradients/s", "shots": k,
+            "path": "drop-in solver (toolbox/solver.py surface), PATH='plain', SU files",
+            "seconds": round(dropin, 2), "misfit": float(J), "finite": bool(np.isfinite(g).all()),
+            "resident": {"value": round(k / resident, 3), "seconds": round(resident, 2),
+                         "note": "plain plan forward + adjoint, inputs on the device"},
+            "batched_over_plain": round(batched_value / (k / dropin), 2),
+            "batched_over_plain_resident": round(batched_value / (k / resident), 2)}
+
+
+def workloads_ricker(w):
+    from swit_amd import workloads
+    return workloads.ricker(w.nt, w.dt, w.f0)
 
 
 def misfit_leg(obs, syn, dt, reps=5):
@@ -198,11 +292,20 @@ def process_leg(syn, dt, offsets, reps=3):
             "traces": int(syn.numel() // syn.shape[-1]), "nt": int(syn.shape[-1])}
 
 
+def host_cores():
+    """CPUs this process may run on (its affinity mask)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        return os.cpu_count() or 1
+
+
 def cpu_baseline(w, src, obs_syn_adj, nshots):
-    """FP32 oracle (C restatement of the reference scheme) on host cores: nshots
-    shot-gradients of the same workload, one thread per shot."""
+    """FP32 oracle (C restatement of the reference scheme) on the host: nshots
+    shot-gradients of the same workload, one thread per shot on every CPU the process
+    may run on."""
     from oracle import oracle as O
-    threads = max(1, min(nshots, os.cpu_count() or 1, 16))
+    threads = max(1, min(nshots, host_cores()))
     rho = w.rho(w.vp_init)
     t0 = time.perf_counter()
     m = O.Model(w.vp_init, rho, w.dx, w.npml, "f32")
@@ -212,8 +315,9 @@ def cpu_baseline(w, src, obs_syn_adj, nshots):
     O.adjoint(m, w.nt, w.dt, w.src_xz[sel], w.rec_xz[sel], src[sel], adj, f, threads=threads)
     dt = time.perf_counter() - t0
     return {"value": nshots / dt, "unit": "shot-gradients/s", "cores": threads, "kind": "port",
-            "sample": f"{nshots} shots of the same workload (full nt), FP32 oracle, one thread per shot, "
-                      f"{dt:.1f} s"}
+            "sample": f"{nshots} of the workload's {w.ns} shots at full nt (observed data given), FP32 oracle "
+                      f"(forward with strips, reconstruction, adjoint, imaging), one thread per shot on the "
+                      f"{threads} CPUs of the process's affinity, {dt:.1f} s"}
 
 
 def main():
@@ -228,11 +332,12 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch configuration")
     ap.add_argument("--no-autograd", action="store_true",
                     help="skip the exact-adjoint (torch.autograd) gradient leg")
-    ap.add_argument("--cpu-shots", type=int, default=0, help="CPU baseline sample size (0: auto)")
+    ap.add_argument("--cpu-shots", type=int, default=0, help="CPU baseline sample size (0: every shot)")
     ap.add_argument("--no-lanes", action="store_true",
                     help="skip the two-lane leg (its half-size launches of the same kernels would mix into a "
                          "profiler's per-kernel averages)")
-    ap.add_argument("--plain-shots", type=int, default=2, help="plain-path comparison sample (0: skip)")
+    ap.add_argument("--plain-shots", type=int, default=-1,
+                    help="plain-path comparison: shots (-1: all of them; 0: skip)")
     ap.add_argument("--iters", type=int, default=10, help="L-BFGS iterations of --workload lbfgs")
     ap.add_argument("--out", default="precision_c5.npz",
                     help="with --workload precision: where the two-shot gradients go")
@@ -366,31 +471,13 @@ def main():
         autograd = autograd_leg(w, src, obs, vp_init, rho_init, args)
 
     plain = None
-    if args.plain_shots > 0 and args.path == "batched":
-        k = min(args.plain_shots, w.ns)
-        pp = Plan(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, w.src_xz[:k], w.rec_xz[:k], path="plain")
-        pp.set_model(vp_true, rho_true)
-        obs_k = pp.forward(src[:k], save_strips=False).clone()
-        torch.cuda.synchronize()
-        # two gradients, the faster one kept: the first also pays the adjoint path's
-        # first-call costs
-        best = float("inf")
-        for _ in range(2):
-            t1 = time.perf_counter()
-            pp.set_model(vp_init, rho_init)
-            syn_k = pp.forward(src[:k], save_strips=True)
-            pp.adjoint(src[:k], syn_k - obs_k)
-            torch.cuda.synchronize()
-            best = min(best, time.perf_counter() - t1)
-        pv = k / best
-        pp.close()
-        plain = {"value": round(pv, 3), "unit": "shot-gradients/s", "shots": k,
-                 "batched_over_plain": round(value / pv, 2)}
+    if args.plain_shots and args.path == "batched":
+        plain = plain_leg(w, src, obs, vp_true, rho_true, vp_init, rho_init, value, args)
 
     cpu = None
     if not args.no_cpu:
         obs_np = obs.cpu().numpy()
-        n = args.cpu_shots or min(16, os.cpu_count() or 1)
+        n = args.cpu_shots or w.ns
         cpu = cpu_baseline(w, src_np, lambda syn: (syn - obs_np[: syn.shape[0]]).astype(np.float32), n)
 
     line = {
