@@ -31,15 +31,16 @@ inner products over two history columns are float32 ones; everything else is flo
   decision  (inversion.py:403-449) NLCG: a failed search after more than one CG step
             clears the CG memory and keeps the model; otherwise (including a failed
             search on the first CG step) the trial model is taken.  L-BFGS: a failed
-            search keeps the model and starts the method afresh; a successful one
+            search keeps the model and re-initialises the whole state, so that the
+            iteration logs 0 trials, step 0 and a previous misfit of 0; a successful one
             takes the trial model
 
 How it is pinned (tests/test_oracle_inversion.py): the two-loop recursion equals the
 dense BFGS inverse-Hessian update built from the same pairs (Nocedal and Wright,
-algorithm 7.4 against eq. 6.17); Polak-Ribiere by its formula; the clipping order;
-on quadratics, NLCG with exact steps reaches the minimum of an n-dimensional problem in
-n steps and L-BFGS lowers the misfit at every accepted step; a forced line-search
-failure takes each scheme's restart path.
+algorithm 7.4 against eq. 6.17); Polak-Ribiere by its formula; the clipping order; on a
+separable quadratic the first step is steepest descent, every accepted model lowers the
+misfit and L-BFGS converges; later L-BFGS steps bound the largest model change to
+[20, 300]; forced line-search failures take each scheme's restart path.
 """
 from __future__ import annotations
 
@@ -189,6 +190,7 @@ class ReferenceLoop:
                 self._reset(self.v.size)
                 self.v_prev = self.v.copy()
                 self.f = f_now
+                tries, step = 0, 0.0          # the re-initialised state logs no search
             else:
                 self.g_prev, self.v_prev = g, self.v
                 self.f, self.v = f_try, v_try

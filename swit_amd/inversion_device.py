@@ -279,9 +279,12 @@ class _State:
             self.d_prev = self.d_prev * 0.0
             print('line search fails: restarting the line search ...')
         elif self.scheme == 'LBFGS' and self.failed:
+            # a fresh start of the whole state (inversion.py:425-434): the logs then show
+            # no line search and a previous misfit of 0
             self.restart()
             self.v_prev = self.v
             self.f_prev = 0.0
+            self.trials, self.step = 0, 0.0
             print('line search fails: restarting the line search ...')
         else:
             self.g_prev, self.d_prev, self.v_prev = self.g, self.d, self.v
