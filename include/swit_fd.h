@@ -268,6 +268,23 @@ int swit_grad_precond_work(const swit_precond *p, size_t *bytes);
 int swit_grad_precond(const swit_precond *p, const float *grad, const float *forw, const float *back,
                       const double *mask, double *out, void *work, size_t work_bytes, hipStream_t stream);
 
+/* Size of the device-resident workspace of swit_grad_precond_noalloc: 64 MiB, enough for
+ * models of up to about 2.09 million cells (swit_grad_precond_work reports the need). */
+#define SWIT_PRECOND_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* swit_grad_precond without a workspace argument and without any allocation, for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight loops, harnesses
+ * that forbid hipMalloc).  It has no counterpart in the reference.  The workspace is a
+ * statically allocated device array of SWIT_PRECOND_SCRATCH_BYTES that each device holds
+ * for the life of the process; its address is resolved once per device, on that device's
+ * first call.  Arguments and results are swit_grad_precond's.  The stream must belong to
+ * the current device (NULL: its default stream).  Returns SWIT_EINVAL when
+ * swit_grad_precond_work exceeds SWIT_PRECOND_SCRATCH_BYTES.  Calls on one device share
+ * the workspace: calls that overlap in time on one device (two streams, two threads)
+ * race, and must be serialised by the caller. */
+int swit_grad_precond_noalloc(const swit_precond *p, const float *grad, const float *forw, const float *back,
+                              const double *mask, double *out, hipStream_t stream);
+
 /* ---- shot-gather processing (toolbox/preprocess.py:23-200) ----------------------------
  * process_workflow_serial's steps on device gathers, in its order: offset mutes and the
  * first-break window mute (apply_mute, :98-151), Butterworth filtfilt followed by

@@ -49,11 +49,13 @@ EXPORTS = (
     "swit_plan_kernel_time",
     "swit_grad_precond_work",
     "swit_grad_precond",
+    "swit_grad_precond_noalloc",
     "swit_process_work",
     "swit_process",
 )
 
 SWIT_MISFIT_WAVEFORM = 0
+SWIT_PRECOND_SCRATCH_BYTES = 64 << 20
 SWIT_MISFIT_GLOBAL_CORRELATION = 1
 SWIT_MISFIT_ENVELOPE = 2
 SWIT_MISFIT_TRAVELTIME = 3
@@ -157,6 +159,8 @@ def load(path: str | None = None):
     L.swit_grad_precond_work.restype = ctypes.c_int
     L.swit_grad_precond.argtypes = [ctypes.POINTER(SwitPrecond), vp, vp, vp, vp, vp, vp, ctypes.c_size_t, vp]
     L.swit_grad_precond.restype = ctypes.c_int
+    L.swit_grad_precond_noalloc.argtypes = [ctypes.POINTER(SwitPrecond), vp, vp, vp, vp, vp, vp]
+    L.swit_grad_precond_noalloc.restype = ctypes.c_int
     L.swit_process_work.argtypes = [ctypes.POINTER(SwitProcessParams), ctypes.POINTER(ctypes.c_size_t)]
     L.swit_process_work.restype = ctypes.c_int
     L.swit_process.argtypes = [ctypes.POINTER(SwitProcessParams), vp, vp, vp, vp, vp, vp, ctypes.c_size_t, vp]

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -1790,18 +1791,21 @@ extern "C" int swit_grad_precond_work(const swit_precond *p, size_t *bytes) {
     return SWIT_OK;
 }
 
-extern "C" int swit_grad_precond(const swit_precond *p, const float *grad, const float *forw, const float *back,
-                                 const double *mask, double *out, void *work, size_t work_bytes,
-                                 hipStream_t stream) {
-    g_err[0] = 0;
-    if (!p || !grad || !forw || !back || !out || !work)
-        return set_err(SWIT_EINVAL, "parameters, grad, forw, back, out and work are required");
+namespace {
+int precond_check(const swit_precond *p, const float *grad, const float *forw, const float *back, const double *out) {
+    if (!p || !grad || !forw || !back || !out)
+        return set_err(SWIT_EINVAL, "parameters, grad, forw, back and out are required");
     if (p->nx < 1 || p->nz < 1) return set_err(SWIT_EINVAL, "nx and nz must be positive");
     if (p->grad_mute < 0 || p->grad_mute > p->nz || p->grad_smooth < 0)
         return set_err(SWIT_EINVAL, "grad_mute must be in [0, nz] and grad_smooth >= 0");
-    size_t need;
-    TRY(swit_grad_precond_work(p, &need));
-    if (work_bytes < need) return set_err(SWIT_EINVAL, "work is %zu bytes, needs %zu", work_bytes, need);
+    return SWIT_OK;
+}
+
+// The preconditioner's launches on a checked parameter set and a workspace of at least
+// swit_grad_precond_work bytes.  Every workspace value is written before it is read
+// (fields, kernel weights, maxima), so the workspace needs no clearing.
+int precond_run(const swit_precond *p, const float *grad, const float *forw, const float *back, const double *mask,
+                double *out, void *work, hipStream_t stream) {
     const int nx = p->nx, nz = p->nz;
     const size_t n = (size_t)nx * nz;
     int sill;
@@ -1866,6 +1870,72 @@ extern "C" int swit_grad_precond(const swit_precond *p, const float *grad, const
     hipLaunchKernelGGL(pc::k_scale, dim3(ge), dim3(256), 0, stream, n, p->vpmax, M + 4, out);
     LAUNCH_CHECK();
     return SWIT_OK;
+}
+
+}  // namespace
+
+// Device-resident workspace of swit_grad_precond_noalloc.  Each device that runs it holds
+// its own copy, for the life of the process; calls on one device share it.
+static __device__ __align__(256) unsigned char scratch[SWIT_PRECOND_SCRATCH_BYTES];
+
+namespace {
+constexpr size_t kPrecondScratchBytes = SWIT_PRECOND_SCRATCH_BYTES;
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kScratchDevices];   // device address of `scratch`, per device
+
+// The current device's address of `scratch`, resolved once per device with that device
+// current.  Host code must use this address: the array's own name is a host stand-in.
+int precond_scratch(hipStream_t stream, void **ptr) {
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchDevices)
+        return set_err(SWIT_EINVAL, "device %d is beyond the %d devices the workspace table holds", dev,
+                       kScratchDevices);
+    if (stream) {
+        hipDevice_t sdev = -1;
+        HIP_TRY(hipStreamGetDevice(stream, &sdev));
+        if (sdev != dev)
+            return set_err(SWIT_EINVAL, "the stream belongs to device %d but device %d is current", (int)sdev, dev);
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *q = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)));
+        if (!q) return set_err(SWIT_EHIP, "hipGetSymbolAddress returned no address for the workspace");
+        g_scratch_ptr[dev] = q;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return SWIT_OK;
+}
+}  // namespace
+
+extern "C" int swit_grad_precond(const swit_precond *p, const float *grad, const float *forw, const float *back,
+                                 const double *mask, double *out, void *work, size_t work_bytes,
+                                 hipStream_t stream) {
+    g_err[0] = 0;
+    TRY(precond_check(p, grad, forw, back, out));
+    if (!work) return set_err(SWIT_EINVAL, "work is required");
+    size_t need;
+    TRY(swit_grad_precond_work(p, &need));
+    if (work_bytes < need) return set_err(SWIT_EINVAL, "work is %zu bytes, needs %zu", work_bytes, need);
+    return precond_run(p, grad, forw, back, mask, out, work, stream);
+}
+
+extern "C" int swit_grad_precond_noalloc(const swit_precond *p, const float *grad, const float *forw,
+                                         const float *back, const double *mask, double *out, hipStream_t stream) {
+    g_err[0] = 0;
+    TRY(precond_check(p, grad, forw, back, out));
+    size_t need;
+    TRY(swit_grad_precond_work(p, &need));
+    if (need > kPrecondScratchBytes)
+        return set_err(SWIT_EINVAL,
+                       "a %d x %d model needs %zu bytes of workspace; swit_grad_precond_noalloc holds %zu "
+                       "(use swit_grad_precond with a workspace)",
+                       p->nx, p->nz, need, kPrecondScratchBytes);
+    void *work = nullptr;
+    TRY(precond_scratch(stream, &work));
+    return precond_run(p, grad, forw, back, mask, out, work, stream);
 }
 
 // ---- shot-gather processing --------------------------------------------------------------
