@@ -107,34 +107,37 @@ __global__ __launch_bounds__(kMisfitThreads) void k_sum_fixed(int n, const doubl
 
 // ---- FFT misfits ------------------------------------------------------------------------
 // One workgroup per trace holds the trace as nt complex doubles in LDS and transforms
-// it in place.  The FFT is a Stockham autosort over the radices of nt (any factor; 2s
-// are paired to 4), written as a gather: each thread computes its outputs of a stage
-// into registers from the stage's inputs, then all threads store them.  Output o of a
-// stage with radix R after ns points of earlier stages is
-//   sum_r x[j + r nt/R] W^(r (jm + q ns) nt/(ns R)),  jm = o mod ns, q = (o/ns) mod R,
-//   j = (o / (ns R)) ns + jm,  W = exp(-+2 pi i / nt),
-// W^(r step) is formed by repeated products from W^step, which comes from two LDS
-// tables of about sqrt(nt) entries each: W^e = A[e / s] B[e mod s].  Two real traces go
-// through one complex transform.
+// it in place.  The FFT is a Stockham autosort over the radices of nt (4s first, then 2,
+// 3 and the odd primes).  A stage of radix R after ns points of earlier stages maps
+// butterfly b (0 <= b < n/R), jm = b mod ns, inputs x[b + q n/R] (q < R), to outputs
+//   y[(b - jm) R + jm + r ns] = sum_q x[b + q n/R] W^(q jm n/(ns R)) w_R^(q r),
+// W = exp(-+2 pi i / n), w_R = W^(n/R).  Radices 2, 3, 4 and 5 run as register
+// butterflies (one thread per butterfly: one table twiddle, its powers by products,
+// R - 1 twiddle products, then the R-point DFT with its constants); larger primes run as a gather in which each thread sums the R
+// inputs of its outputs with the twiddle powers formed by repeated products.  Twiddles
+// come from two LDS tables: W^e = A[e >> kTwShift] B[e & (kTwB - 1)].  Two real traces
+// go through one complex transform.
 constexpr int kFftMaxStages = 24;
 constexpr int kFftMaxN = 9216;
-constexpr int kFftTw = 96;      // ceil(sqrt(kFftMaxN)) and ceil(kFftMaxN / that)
+constexpr int kTwShift = 7;
+constexpr int kTwB = 1 << kTwShift;                          // 128
+constexpr int kTwA = (kFftMaxN + kTwB - 1) / kTwB;           // 72
 
 struct FftFactors {
-    int n, nst, s;              // length, stages, twiddle split (s = ceil(sqrt(n)))
+    int n, nst, s;              // length, stages, twiddle split (s = kTwB)
     int radix[kFftMaxStages];
 };
 
-// Threads per workgroup for a transform of at most NMAX points: the largest size runs
-// at 512 so that its per-thread outputs fit in registers.
+// Threads per workgroup: 512, so that two workgroups of up to 4096 points share a CU
+// (each barrier of one overlaps the other's work).
 template <int NMAX>
-constexpr int fft_threads() { return NMAX > 4096 ? 512 : 1024; }
+constexpr int fft_threads() { return 512; }
 
 template <int NMAX>
 struct FftLds {
     static constexpr int T = fft_threads<NMAX>();
     double2 x[NMAX];
-    double2 ta[kFftTw], tb[kFftTw];      // exp(-2 pi i a s / n), exp(-2 pi i b / n)
+    double2 ta[kTwA], tb[kTwB];      // exp(-2 pi i a kTwB / n), exp(-2 pi i b / n)
     double sh[T / 64];
     int ish[T / 64];
 };
@@ -142,9 +145,9 @@ struct FftLds {
 template <int NMAX>
 __device__ void fft_twiddles(FftLds<NMAX> &L, const FftFactors &f) {
     constexpr int T = fft_threads<NMAX>();
-    const int na = (f.n + f.s - 1) / f.s;
-    for (int i = threadIdx.x; i < na + f.s; i += T) {
-        const int e = i < na ? i * f.s : i - na;
+    const int na = (f.n + kTwB - 1) / kTwB;
+    for (int i = threadIdx.x; i < na + kTwB; i += T) {
+        const int e = i < na ? i * kTwB : i - na;
         double sv, cv;
         sincospi(2.0 * (double)e / (double)f.n, &sv, &cv);
         if (i < na) L.ta[i] = make_double2(cv, -sv);
@@ -152,51 +155,162 @@ __device__ void fft_twiddles(FftLds<NMAX> &L, const FftFactors &f) {
     }
 }
 
+// W^e (conjugated for the inverse), 0 <= e < n
+template <int NMAX, bool INV>
+__device__ __forceinline__ double2 fft_w(const FftLds<NMAX> &L, int e) {
+    const double2 a = L.ta[e >> kTwShift], b = L.tb[e & (kTwB - 1)];
+    const double re = a.x * b.x - a.y * b.y, im = a.x * b.y + a.y * b.x;
+    return make_double2(re, INV ? -im : im);
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double c) { return make_double2(c * a.x, c * a.y); }
+// -i a (forward sign) or +i a (inverse)
+template <bool INV> __device__ __forceinline__ double2 rot(double2 a) {
+    return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+
+// R-point DFT of z into y with the forward (-) or inverse (+) sign.
+template <int R, bool INV> __device__ __forceinline__ void dft_small(const double2 *z, double2 *y);
+template <bool INV> __device__ __forceinline__ void dft_small2(const double2 *z, double2 *y) {
+    y[0] = cadd(z[0], z[1]);
+    y[1] = csub(z[0], z[1]);
+}
+template <bool INV> __device__ __forceinline__ void dft_small3(const double2 *z, double2 *y) {
+    const double h = 0.86602540378443864676;                       // sin(2 pi / 3)
+    const double2 t1 = cadd(z[1], z[2]);
+    const double2 t2 = csub(z[0], cscale(t1, 0.5));
+    const double2 t3 = rot<INV>(cscale(csub(z[1], z[2]), h));
+    y[0] = cadd(z[0], t1);
+    y[1] = cadd(t2, t3);
+    y[2] = csub(t2, t3);
+}
+template <bool INV> __device__ __forceinline__ void dft_small4(const double2 *z, double2 *y) {
+    const double2 a = cadd(z[0], z[2]), b = csub(z[0], z[2]);
+    const double2 c = cadd(z[1], z[3]), d = rot<INV>(csub(z[1], z[3]));
+    y[0] = cadd(a, c);
+    y[1] = cadd(b, d);
+    y[2] = csub(a, c);
+    y[3] = csub(b, d);
+}
+template <bool INV> __device__ __forceinline__ void dft_small5(const double2 *z, double2 *y) {
+    const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;   // cos(2 pi k / 5)
+    const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;    // sin(2 pi k / 5)
+    const double2 a1 = cadd(z[1], z[4]), a2 = cadd(z[2], z[3]);
+    const double2 b1 = csub(z[1], z[4]), b2 = csub(z[2], z[3]);
+    const double2 t1 = make_double2(z[0].x + c1 * a1.x + c2 * a2.x, z[0].y + c1 * a1.y + c2 * a2.y);
+    const double2 t2 = make_double2(z[0].x + c2 * a1.x + c1 * a2.x, z[0].y + c2 * a1.y + c1 * a2.y);
+    const double2 u1 = rot<INV>(make_double2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
+    const double2 u2 = rot<INV>(make_double2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
+    y[0] = cadd(z[0], cadd(a1, a2));
+    y[1] = cadd(t1, u1);
+    y[4] = csub(t1, u1);
+    y[2] = cadd(t2, u2);
+    y[3] = csub(t2, u2);
+}
+template <> __device__ __forceinline__ void dft_small<2, false>(const double2 *z, double2 *y) { dft_small2<false>(z, y); }
+template <> __device__ __forceinline__ void dft_small<2, true>(const double2 *z, double2 *y) { dft_small2<true>(z, y); }
+template <> __device__ __forceinline__ void dft_small<3, false>(const double2 *z, double2 *y) { dft_small3<false>(z, y); }
+template <> __device__ __forceinline__ void dft_small<3, true>(const double2 *z, double2 *y) { dft_small3<true>(z, y); }
+template <> __device__ __forceinline__ void dft_small<4, false>(const double2 *z, double2 *y) { dft_small4<false>(z, y); }
+template <> __device__ __forceinline__ void dft_small<4, true>(const double2 *z, double2 *y) { dft_small4<true>(z, y); }
+template <> __device__ __forceinline__ void dft_small<5, false>(const double2 *z, double2 *y) { dft_small5<false>(z, y); }
+template <> __device__ __forceinline__ void dft_small<5, true>(const double2 *z, double2 *y) { dft_small5<true>(z, y); }
+
+// One butterfly stage of radix R (2..5), in place: all reads, barrier, all writes, barrier.
+template <int NMAX, bool INV, int R>
+__device__ void fft_stage_bfly(FftLds<NMAX> &L, int n, int ns) {
+    constexpr int T = fft_threads<NMAX>();
+    constexpr int BPT = (NMAX / R + T - 1) / T;      // butterflies per thread
+    const int nb = n / R, mult = nb / ns;            // n / (ns R)
+    double2 y[BPT][R];
+    int o0[BPT];
+#pragma unroll
+    for (int k = 0; k < BPT; ++k) {
+        const int b = threadIdx.x + k * T;
+        o0[k] = -1;
+        if (b < nb) {
+            const int jm = b % ns;
+            o0[k] = (b - jm) * R + jm;
+            // twiddle powers w^q from w = W^(jm n/(ns R)) by products
+            double2 z[R], wq[R];
+            wq[1] = fft_w<NMAX, INV>(L, jm * mult);
+#pragma unroll
+            for (int q = 2; q < R; ++q) wq[q] = cmul(wq[q - 1], wq[1]);
+            z[0] = L.x[b];
+#pragma unroll
+            for (int q = 1; q < R; ++q) z[q] = cmul(L.x[b + q * nb], wq[q]);
+            dft_small<R, INV>(z, y[k]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < BPT; ++k) {
+        if (o0[k] >= 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) L.x[o0[k] + r * ns] = y[k][r];
+        }
+    }
+    __syncthreads();
+}
+
+// One stage of any radix as a gather over outputs, in place (reads, barrier, writes,
+// barrier).
+template <int NMAX, bool INV>
+__device__ void fft_stage_gather(FftLds<NMAX> &L, int n, int R, int ns) {
+    constexpr int T = fft_threads<NMAX>(), E = (NMAX + T - 1) / T;
+    const int nr = n / R, span = ns * R, mult = n / span;
+    double2 acc[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int o = threadIdx.x + k * T;
+        double re = 0.0, im = 0.0;
+        if (o < n) {
+            const int jm = o % ns, q = (o / ns) % R;
+            const int j = (o / span) * ns + jm;
+            const double2 w1 = fft_w<NMAX, INV>(L, (jm + q * ns) * mult);     // step < n
+            double wr = 1.0, wi = 0.0;
+            for (int r = 0; r < R; ++r) {
+                const double2 v = L.x[j + r * nr];
+                re = fma(v.x, wr, fma(-v.y, wi, re));
+                im = fma(v.x, wi, fma(v.y, wr, im));
+                const double t = fma(wr, w1.x, -wi * w1.y);
+                wi = fma(wr, w1.y, wi * w1.x);
+                wr = t;
+            }
+        }
+        acc[k] = make_double2(re, im);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int o = threadIdx.x + k * T;
+        if (o < n) L.x[o] = acc[k];
+    }
+    __syncthreads();
+}
+
 // In-place DFT of L.x[0..n) (INV: inverse without the 1/n factor).  Begins and ends
 // with a barrier.
 template <int NMAX, bool INV>
 __device__ void fft_lds(FftLds<NMAX> &L, const FftFactors &f) {
-    constexpr int T = fft_threads<NMAX>(), E = (NMAX + T - 1) / T;
-    const int n = f.n, s = f.s;
+    const int n = f.n;
     int ns = 1;
     __syncthreads();
     for (int st = 0; st < f.nst; ++st) {
-        const int R = f.radix[st], nr = n / R, span = ns * R, mult = n / span;
-        double2 acc[E];
-#pragma unroll
-        for (int k = 0; k < E; ++k) {
-            const int o = threadIdx.x + k * T;
-            double re = 0.0, im = 0.0;
-            if (o < n) {
-                const int jm = o % ns, q = (o / ns) % R;
-                const int j = (o / span) * ns + jm;
-                const int step = (jm + q * ns) * mult;        // < n
-                // w1 = W^step from the tables; W^(r step) by repeated products
-                const double2 a = L.ta[step / s], b = L.tb[step % s];
-                const double w1r = a.x * b.x - a.y * b.y;
-                const double w1i = INV ? -(a.x * b.y + a.y * b.x) : a.x * b.y + a.y * b.x;
-                // fused multiply-adds (explicit: the library is built without
-                // contraction, which the stencils need; here it only removes roundings)
-                double wr = 1.0, wi = 0.0;
-                for (int r = 0; r < R; ++r) {
-                    const double2 v = L.x[j + r * nr];
-                    re = fma(v.x, wr, fma(-v.y, wi, re));
-                    im = fma(v.x, wi, fma(v.y, wr, im));
-                    const double t = fma(wr, w1r, -wi * w1i);
-                    wi = fma(wr, w1i, wi * w1r);
-                    wr = t;
-                }
-            }
-            acc[k] = make_double2(re, im);
+        const int R = f.radix[st];
+        switch (R) {
+        case 2: fft_stage_bfly<NMAX, INV, 2>(L, n, ns); break;
+        case 3: fft_stage_bfly<NMAX, INV, 3>(L, n, ns); break;
+        case 4: fft_stage_bfly<NMAX, INV, 4>(L, n, ns); break;
+        case 5: fft_stage_bfly<NMAX, INV, 5>(L, n, ns); break;
+        default: fft_stage_gather<NMAX, INV>(L, n, R, ns); break;
         }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < E; ++k) {
-            const int o = threadIdx.x + k * T;
-            if (o < n) L.x[o] = acc[k];
-        }
-        __syncthreads();
-        ns = span;
+        ns *= R;
     }
 }
 
@@ -215,7 +329,8 @@ __device__ void hilbert_multiplier(FftLds<NMAX> &L, int n) {
 
 // Envelope misfit (power 2) of one trace per workgroup.
 template <int NMAX>
-__global__ __launch_bounds__(fft_threads<NMAX>()) void k_misfit_envelope(FftFactors f, float dt,
+__global__ __launch_bounds__(fft_threads<NMAX>())
+__attribute__((amdgpu_waves_per_eu(NMAX <= 4096 ? 4 : 1, 8))) void k_misfit_envelope(FftFactors f, float dt,
                                                                  const float *__restrict__ obs,
                                                                  const float *__restrict__ syn,
                                                                  float *__restrict__ adj,
@@ -275,7 +390,8 @@ __global__ __launch_bounds__(fft_threads<NMAX>()) void k_misfit_envelope(FftFact
 
 // Cross-correlation traveltime misfit of one trace per workgroup.
 template <int NMAX>
-__global__ __launch_bounds__(fft_threads<NMAX>()) void k_misfit_traveltime(FftFactors f, float dt,
+__global__ __launch_bounds__(fft_threads<NMAX>())
+__attribute__((amdgpu_waves_per_eu(NMAX <= 4096 ? 4 : 1, 8))) void k_misfit_traveltime(FftFactors f, float dt,
                                                                    const float *__restrict__ obs,
                                                                    const float *__restrict__ syn,
                                                                    float *__restrict__ adj,

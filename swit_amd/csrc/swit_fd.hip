@@ -1464,8 +1464,7 @@ extern "C" int swit_group_adjoint(swit_group *gr, const float *const *src, const
 static FftFactors fft_factors(int n) {
     FftFactors f{};
     f.n = n;
-    f.s = 1;
-    while (f.s * f.s < n) ++f.s;
+    f.s = kTwB;
     int m = n;
     for (int p : {4, 2, 3})
         while (m % p == 0) { f.radix[f.nst++] = p; m /= p; }
