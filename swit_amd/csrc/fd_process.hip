@@ -21,6 +21,7 @@ namespace pr {
 
 constexpr int kThreads = 256;
 constexpr int kFilterThreads = 64;
+constexpr int kFilterBlock = 16;     // samples per block of loads in k_filtfilt
 constexpr int kMaxTaps = 16;
 constexpr int kMuteLength = 100;    // samples of the arrival-mute taper (preprocess.py:101)
 
@@ -160,19 +161,36 @@ __global__ __launch_bounds__(kFilterThreads) void k_filtfilt(ProcArgs p, int ntr
     float *x = data + (size_t)tr * nt;
     double z[K];
     z[K - 1] = 0.0;
-    // forward: y = lfilter(b, a, ext, zi = zi * ext[0])
+    // forward: y = lfilter(b, a, ext, zi = zi * ext[0]).  Samples go in blocks of
+    // kFilterBlock: a block's loads are issued together, then the recursion runs over
+    // them, so the load latency is paid once per block rather than once per sample.
     const double x0 = (double)odd_ext(x, 0, pad, nt);
 #pragma unroll
     for (int k = 0; k < K - 1; ++k) z[k] = p.zi[k] * x0;
-    for (int n = 0; n < L; ++n) y[(size_t)n * ntraces + tr] = df2t_step<K>((double)odd_ext(x, n, pad, nt), z, p);
+    for (int n0 = 0; n0 < L; n0 += kFilterBlock) {
+        double xb[kFilterBlock];
+#pragma unroll
+        for (int i = 0; i < kFilterBlock; ++i) xb[i] = n0 + i < L ? (double)odd_ext(x, n0 + i, pad, nt) : 0.0;
+#pragma unroll
+        for (int i = 0; i < kFilterBlock; ++i)
+            if (n0 + i < L) y[(size_t)(n0 + i) * ntraces + tr] = df2t_step<K>(xb[i], z, p);
+    }
     // backward over the reversed output from zi * y[L - 1]; step n of this pass is
     // time L - 1 - n of the extended trace
     const double y0 = y[(size_t)(L - 1) * ntraces + tr];
 #pragma unroll
     for (int k = 0; k < K - 1; ++k) z[k] = p.zi[k] * y0;
-    for (int t = L - 1; t >= 0; --t) {
-        const double yn = df2t_step<K>(y[(size_t)t * ntraces + tr], z, p);
-        if (t >= pad && t < pad + nt) x[t - pad] = (float)yn;
+    for (int t0 = L - 1; t0 >= 0; t0 -= kFilterBlock) {
+        double yb[kFilterBlock];
+#pragma unroll
+        for (int i = 0; i < kFilterBlock; ++i) yb[i] = t0 - i >= 0 ? y[(size_t)(t0 - i) * ntraces + tr] : 0.0;
+#pragma unroll
+        for (int i = 0; i < kFilterBlock; ++i) {
+            const int t = t0 - i;
+            if (t < 0) break;
+            const double yn = df2t_step<K>(yb[i], z, p);
+            if (t >= pad && t < pad + nt) x[t - pad] = (float)yn;
+        }
     }
 }
 
