@@ -156,6 +156,26 @@ def lanes_leg(w, src, obs, vp, rho, g_ref, args, lanes=2):
             "autotuned": [t.get("bwd_steps") for t in tuned] if tuned else None}
 
 
+def two_streams_leg(w, plan, step, g_ref, args):
+    """The main plan with SWIT_OPT_STREAMS 2 (each half of the shots runs its time loops
+    on a stream of its own): timed as the main steps; the gradient and illuminations
+    must equal the one-stream step's bit for bit."""
+    import torch
+    plan.set_option("streams", 2)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g2 = step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    plan.set_option("streams", 1)
+    same = all(bool(torch.equal(a, b)) for a, b in zip(g2, g_ref))
+    return {"value": round(w.ns * args.steps / el, 3), "unit": "shot-gradients/s",
+            "ms_per_step": round(1000 * el / args.steps, 2), "identical_to_one_stream": same}
+
+
 def plain_leg(w, src, obs, vp_true, rho_true, vp_init, rho_init, batched_value, args):
     """The plain per-shot GPU path (one thread per cell, one launch per sweep, shot by
     shot), two ways, outside the timed steps:
@@ -334,8 +354,8 @@ def main():
                     help="skip the exact-adjoint (torch.autograd) gradient leg")
     ap.add_argument("--cpu-shots", type=int, default=0, help="CPU baseline sample size (0: every shot)")
     ap.add_argument("--no-lanes", action="store_true",
-                    help="skip the two-lane leg (its half-size launches of the same kernels would mix into a "
-                         "profiler's per-kernel averages)")
+                    help="skip the two-lane and two-stream legs (their half-size launches of the same kernels "
+                         "would mix into a profiler's per-kernel averages)")
     ap.add_argument("--plain-shots", type=int, default=-1,
                     help="plain-path comparison: shots (-1: all of them; 0: skip)")
     ap.add_argument("--iters", type=int, default=10, help="L-BFGS iterations of --workload lbfgs")
@@ -457,6 +477,8 @@ def main():
 
     lanes = (lanes_leg(w, src, obs, vp_init, rho_init, g[0], args)
              if args.path == "batched" and not args.no_lanes else None)
+    two_streams = (two_streams_leg(w, plan, step, g, args)
+                   if args.path == "batched" and not args.no_lanes else None)
     misfits = processing = None
     if args.path == "batched":
         syn_now = plan.forward(src, save_strips=False)
@@ -501,6 +523,7 @@ def main():
         "plain_baseline": plain,
         "autograd": autograd,
         "concurrent_lanes": lanes,
+        "two_streams": two_streams,
         "device_misfits": misfits,
         "device_processing": processing,
     }

@@ -139,6 +139,12 @@ const char *swit_last_error(void);
                                    two receivers share a cell)                        */
 #define SWIT_OPT_BWD_TILE_X 128 /* tile width of the one-step backward kernel: 24
                                    (default) or 16 columns (same results)             */
+#define SWIT_OPT_STREAMS 256    /* 1 (default); 2: the first and second half of the shots
+                                   run their forward and reverse time loops on two
+                                   streams (the caller's and the plan's own), forked
+                                   from and joined back into the caller's stream, so
+                                   that each step's two launches overlap. Results are
+                                   bit-identical to one stream.                        */
 int swit_plan_set_option(swit_plan *plan, int option, int value);
 
 /* Autotuner (no counterpart in the reference): times `steps` back-to-back launches of

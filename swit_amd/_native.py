@@ -67,6 +67,7 @@ SWIT_OPT_BWD_COLGROUP = 16
 SWIT_OPT_BWD_STEPS = 32
 SWIT_OPT_EXACT_FUSED = 64
 SWIT_OPT_BWD_TILE_X = 128
+SWIT_OPT_STREAMS = 256
 
 
 class SwitPrecond(ctypes.Structure):

@@ -182,12 +182,12 @@ __device__ __forceinline__ float wave_max(float v) {
 constexpr int kEncThreads = 256;
 constexpr int kStripRing = 32;     // rows of the FP32 ring; encoded in batches of up to this many
 template <int SM>
-__global__ __launch_bounds__(kEncThreads) void k_strip_encode(Dims d, int first_row, int nring,
+__global__ __launch_bounds__(kEncThreads) void k_strip_encode(Dims d, int first_row, int nring, int s0,
                                                               const float *__restrict__ ring,
                                                               uint32_t *__restrict__ smax,
                                                               uint16_t *__restrict__ strips16) {
     __shared__ float sh[kEncThreads / 64];
-    const int s = blockIdx.y, row = first_row + blockIdx.x;
+    const int s = s0 + (int)blockIdx.y, row = first_row + blockIdx.x;
     const float *v = ring + ((size_t)s * nring + row % nring) * d.S;
     float m = 0.0f;
     for (int i = threadIdx.x; i < d.S; i += kEncThreads) m = fmaxf(m, fabsf(v[i]));
@@ -279,7 +279,7 @@ struct DSave {
 // strips[(s * s_rows + s_row) * S] (the plan's strips with s_rows = nt, s_row = it - 1, or
 // a ring of rows that k_strip_encode turns into 16-bit rows).
 __global__ __launch_bounds__(kThreads) void k_fwd_fused(
-    Dims d, int it, TileMap tm, int ns, int G, const float2 *__restrict__ CP, const float4 *__restrict__ CV,
+    Dims d, int it, TileMap tm, int s0, int ns, int G, const float2 *__restrict__ CP, const float4 *__restrict__ CV,
     const ShotGeom *__restrict__ geo, const float *__restrict__ src, const float *__restrict__ In,
     float *__restrict__ Out, const int *__restrict__ rec_start, const RecRef *__restrict__ rec_list,
     float *__restrict__ seis_t, const int *__restrict__ strip_start, const StripRef *__restrict__ strip_list,
@@ -290,8 +290,8 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
     __shared__ float sW[kNR];
     __shared__ float sQ[kNR];     // new p
 
-    // the ns shots in ceil(ns / G) groups of near-equal size (at most G), so that every
-    // workgroup of the launch carries about the same work
+    // shots s0 .. s0 + ns - 1 in ceil(ns / G) groups of near-equal size (at most G), so
+    // that every workgroup of the launch carries about the same work
     const int ngr = (ns + G - 1) / G;
     int grp, t;
     work_item(xcd_logical(blockIdx.x, gridDim.x), ngr, tm, grp, t);
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
     const int tid = threadIdx.x;
     const int fs = d.fs;
     const int np = d.nxp * d.ld;
-    const int s_begin = grp * ns / ngr, s_end = (grp + 1) * ns / ngr;
+    const int s_begin = s0 + grp * ns / ngr, s_end = s0 + (grp + 1) * ns / ngr;
 
     // staged fields of the first shot
     float lp[kLPT], lu[kLPT], lw[kLPT];
@@ -461,7 +461,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
 // backward illumination), updated in the reference's order.  TX: tile width (columns).
 template <int SM, int TX>
 __global__ __launch_bounds__(kThreads) void k_bwd_fused(
-    Dims d, int it, TileMap tm, const float2 *__restrict__ KB, const float2 *__restrict__ CP,
+    Dims d, int it, TileMap tm, int s0, const float2 *__restrict__ KB, const float2 *__restrict__ CP,
     const float4 *__restrict__ CV,
     const ShotGeom *__restrict__ geo, const float *__restrict__ src, const void *__restrict__ strips,
     const uint32_t *__restrict__ smax,
@@ -480,6 +480,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
 
     int s, t;
     work_item(xcd_logical(blockIdx.x, gridDim.x), (int)(gridDim.x / tm.ntiles), tm, s, t);
+    s += s0;                      // the launch covers shots s0 .. s0 + gridDim.x / ntiles - 1
     const TileBox tb = tile_box(d, tm, t);
     const int zo = tb.z0 - kHalo, xo = tb.x0 - kHalo;
     const int lane = threadIdx.x & 63;
@@ -748,7 +749,7 @@ __device__ __forceinline__ int LW(int c, int l) { return c * kZL + l; }
 
 template <int SM>
 __global__ __launch_bounds__(kThreads) void k_bwd_fused2(
-    Dims d, int it, TileMap tm, const float2 *__restrict__ KB, const float2 *__restrict__ CP,
+    Dims d, int it, TileMap tm, int s0, const float2 *__restrict__ KB, const float2 *__restrict__ CP,
     const float4 *__restrict__ CV,
     const ShotGeom *__restrict__ geo, const float *__restrict__ src, const void *__restrict__ strips,
     const uint32_t *__restrict__ smax,
@@ -767,6 +768,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused2(
 
     int s, t;
     work_item(xcd_logical(blockIdx.x, gridDim.x), (int)(gridDim.x / tm.ntiles), tm, s, t);
+    s += s0;                      // the launch covers shots s0 .. s0 + gridDim.x / ntiles - 1
     const TileBox tb = tile_box(d, tm, t);
     const int zo = tb.z0 - kM, xo = tb.x0 - kM;
     const int lane = threadIdx.x & 63;
@@ -1133,6 +1135,13 @@ struct BatchedState {
     hipStream_t cap = nullptr;   // capture stream
     hipGraphExec_t gfwd[2] = {nullptr, nullptr};
     hipGraphExec_t gbwd = nullptr;
+    // two streams (SWIT_OPT_STREAMS 2): each half of the shots runs its time loops on a
+    // stream of its own, from graphs of its own ([half][save] and [half])
+    int streams = 1;
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    hipGraphExec_t gfwd_h[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    hipGraphExec_t gbwd_h[2] = {nullptr, nullptr};
     int use_graph = 1;
     int gen = 0;                 // bumped when captured launch arguments change
     int profile = 0;             // bracket each time loop with a pair of events

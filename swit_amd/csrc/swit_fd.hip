@@ -793,6 +793,11 @@ static void batched_release(swit_plan *pl) {
     for (auto &g : bs.gfwd)
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
     if (bs.gbwd) { (void)hipGraphExecDestroy(bs.gbwd); bs.gbwd = nullptr; }
+    for (auto &row : bs.gfwd_h)
+        for (auto &g : row)
+            if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    for (auto &g : bs.gbwd_h)
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
     if (bs.gex_f) { (void)hipGraphExecDestroy(bs.gex_f); bs.gex_f = nullptr; }
     if (bs.gex_b) { (void)hipGraphExecDestroy(bs.gex_b); bs.gex_b = nullptr; }
     for (auto &t : bs.pending) { (void)hipEventDestroy(t.e0); (void)hipEventDestroy(t.e1); }
@@ -800,6 +805,9 @@ static void batched_release(swit_plan *pl) {
     for (hipEvent_t e : bs.ev_pool) (void)hipEventDestroy(e);
     bs.ev_pool.clear();
     if (bs.cap) { (void)hipStreamDestroy(bs.cap); bs.cap = nullptr; }
+    if (bs.side) { (void)hipStreamDestroy(bs.side); bs.side = nullptr; }
+    if (bs.fork) { (void)hipEventDestroy(bs.fork); bs.fork = nullptr; }
+    if (bs.join) { (void)hipEventDestroy(bs.join); bs.join = nullptr; }
     bs.ready = 0;
 }
 
@@ -861,38 +869,47 @@ static int forward_prologue(swit_plan *pl, int save, hipStream_t st) {
 
 // Forward step it.  With strip saving the step's FP32 strip row goes to the plan's
 // strips (FP32 plans) or to the ring that k_strip_encode empties (16-bit plans).
+// A launch's shots: s0 .. s0 + n - 1 (n < 0: all of the plan's shots).
+struct ShotRange {
+    int s0, n;
+};
+static ShotRange shots_of(const swit_plan *pl, ShotRange r) { return r.n < 0 ? ShotRange{0, pl->ns} : r; }
+
 static void launch_fwd(swit_plan *pl, int it, const float *in, float *out, int save, hipStream_t st,
-                       DSave dsave = DSave{nullptr, 0, 0}, const float *src = nullptr) {
+                       DSave dsave = DSave{nullptr, 0, 0}, const float *src = nullptr,
+                       ShotRange range = ShotRange{0, -1}) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
+    const ShotRange r = shots_of(pl, range);
     float *strips = !save ? nullptr : bs.strip_mode == 0 ? pl->strips : bs.ring;
     const int s_rows = bs.strip_mode == 0 ? d.nt : kStripRing;
     const int s_row = bs.strip_mode == 0 ? it - 1 : (it - 1) % kStripRing;
-    hipLaunchKernelGGL(k_fwd_fused, dim3(bs.ngroups_f * bs.fwd.ntiles), dim3(kThreads), 0, st, d, it, bs.fwd,
-                       pl->ns, bs.Gf, bs.CP, bs.CV, pl->geo, src ? src : bs.src, in, out, bs.rec_start, bs.rec_list,
-                       pl->seis_t, bs.strip_start, bs.strip_list, strips, s_rows, s_row, dsave);
+    hipLaunchKernelGGL(k_fwd_fused, dim3(cdiv(r.n, bs.Gf) * bs.fwd.ntiles), dim3(kThreads), 0, st, d, it, bs.fwd,
+                       r.s0, r.n, bs.Gf, bs.CP, bs.CV, pl->geo, src ? src : bs.src, in, out, bs.rec_start,
+                       bs.rec_list, pl->seis_t, bs.strip_start, bs.strip_list, strips, s_rows, s_row, dsave);
 }
 
 // 16-bit plans: encode strip rows first..last (at most kStripRing, all still in the ring).
-static void launch_encode(swit_plan *pl, int first, int last, hipStream_t st) {
+static void launch_encode(swit_plan *pl, int first, int last, hipStream_t st, ShotRange range = ShotRange{0, -1}) {
     BatchedState &bs = pl->bat;
-    const dim3 grd(last - first + 1, pl->ns);
+    const ShotRange r = shots_of(pl, range);
+    const dim3 grd(last - first + 1, r.n);
     if (bs.strip_mode == 1)
-        hipLaunchKernelGGL(k_strip_encode<1>, grd, dim3(kEncThreads), 0, st, pl->d, first, kStripRing, bs.ring,
-                           bs.smax, bs.strips16);
+        hipLaunchKernelGGL(k_strip_encode<1>, grd, dim3(kEncThreads), 0, st, pl->d, first, kStripRing, r.s0,
+                           bs.ring, bs.smax, bs.strips16);
     else
-        hipLaunchKernelGGL(k_strip_encode<2>, grd, dim3(kEncThreads), 0, st, pl->d, first, kStripRing, bs.ring,
-                           bs.smax, bs.strips16);
+        hipLaunchKernelGGL(k_strip_encode<2>, grd, dim3(kEncThreads), 0, st, pl->d, first, kStripRing, r.s0,
+                           bs.ring, bs.smax, bs.strips16);
 }
 
 template <int SM>
 static void launch_bwd(swit_plan *pl, int it, const float *ri, float *ro, const float *bi, float *bo,
-                       hipStream_t st) {
+                       hipStream_t st, ShotRange r) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
     const void *strips = SM == 0 ? static_cast<const void *>(pl->strips) : static_cast<const void *>(bs.strips16);
     auto kern = bs.bwd.tx == bk::kBTXs[0] ? k_bwd_fused<SM, bk::kBTXs[0]> : k_bwd_fused<SM, bk::kBTXs[1]>;
-    hipLaunchKernelGGL(kern, dim3(pl->ns * bs.bwd.ntiles), dim3(kThreads), 0, st, d, it, bs.bwd, bs.KB, bs.CP,
+    hipLaunchKernelGGL(kern, dim3(r.n * bs.bwd.ntiles), dim3(kThreads), 0, st, d, it, bs.bwd, r.s0, bs.KB, bs.CP,
                        bs.CV, pl->geo, bs.src, strips, bs.smax, ri, ro, bi, bo, bs.inj_start, bs.inj_groups,
                        bs.inj_list, pl->sadj_t, bs.acc);
 }
@@ -911,11 +928,11 @@ static void select_bwd_width(BatchedState &bs, int wi) {
 
 template <int SM>
 static void launch_bwd2(swit_plan *pl, int it, const float *ri, float *ro, const float *bi, float *bo,
-                        hipStream_t st) {
+                        hipStream_t st, ShotRange r) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
     const void *strips = SM == 0 ? static_cast<const void *>(pl->strips) : static_cast<const void *>(bs.strips16);
-    hipLaunchKernelGGL(k_bwd_fused2<SM>, dim3(pl->ns * bs.bwd2.ntiles), dim3(kThreads), 0, st, d, it, bs.bwd2,
+    hipLaunchKernelGGL(k_bwd_fused2<SM>, dim3(r.n * bs.bwd2.ntiles), dim3(kThreads), 0, st, d, it, bs.bwd2, r.s0,
                        bs.KB, bs.CP, bs.CV, pl->geo, bs.src, strips, bs.smax, ri, ro, bi, bo, bs.inj2_start,
                        bs.inj2_groups, bs.inj2_list, bs.injx_start, bs.injx_groups, bs.injx_list, bs.injx_cap,
                        pl->sadj_t, bs.acc);
@@ -924,29 +941,30 @@ static void launch_bwd2(swit_plan *pl, int it, const float *ri, float *ro, const
 // Backward launch that takes the reverse pass from step it down: two steps (it, it-1)
 // when the plan runs the two-step kernel and it >= 3, else one.  Returns the steps done.
 static int launch_backward(swit_plan *pl, int it, const float *ri, float *ro, const float *bi, float *bo,
-                           hipStream_t st) {
+                           hipStream_t st, ShotRange range = ShotRange{0, -1}) {
     BatchedState &bs = pl->bat;
+    const ShotRange r = shots_of(pl, range);
     const bool two = bs.bwd_steps == 2 && it >= 3;
     switch (bs.strip_mode) {
-    case 0: two ? launch_bwd2<0>(pl, it, ri, ro, bi, bo, st) : launch_bwd<0>(pl, it, ri, ro, bi, bo, st); break;
-    case 1: two ? launch_bwd2<1>(pl, it, ri, ro, bi, bo, st) : launch_bwd<1>(pl, it, ri, ro, bi, bo, st); break;
-    default: two ? launch_bwd2<2>(pl, it, ri, ro, bi, bo, st) : launch_bwd<2>(pl, it, ri, ro, bi, bo, st); break;
+    case 0: two ? launch_bwd2<0>(pl, it, ri, ro, bi, bo, st, r) : launch_bwd<0>(pl, it, ri, ro, bi, bo, st, r); break;
+    case 1: two ? launch_bwd2<1>(pl, it, ri, ro, bi, bo, st, r) : launch_bwd<1>(pl, it, ri, ro, bi, bo, st, r); break;
+    default: two ? launch_bwd2<2>(pl, it, ri, ro, bi, bo, st, r) : launch_bwd<2>(pl, it, ri, ro, bi, bo, st, r); break;
     }
     return two ? 2 : 1;
 }
 
-static int forward_loop(swit_plan *pl, int save, hipStream_t st) {
+static int forward_loop(swit_plan *pl, int save, hipStream_t st, ShotRange r = ShotRange{0, -1}) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
     int first = 1;                               // oldest strip row not yet encoded
     for (int it = 2; it <= d.nt; ++it) {
         const float *in = bs.F[it % 2];          // it = 2 reads F[0]
         float *out = bs.F[(it + 1) % 2];
-        launch_fwd(pl, it, in, out, save, st);
+        launch_fwd(pl, it, in, out, save, st, DSave{nullptr, 0, 0}, nullptr, r);
         LAUNCH_CHECK();
         // 16-bit plans: encode the ring once its last slot is written, and at the end
         if (save && bs.strip_mode && ((it - 1) % kStripRing == kStripRing - 1 || it == d.nt)) {
-            launch_encode(pl, first, it - 1, st);
+            launch_encode(pl, first, it - 1, st, r);
             LAUNCH_CHECK();
             first = it;
         }
@@ -977,11 +995,11 @@ static int adjoint_prologue(swit_plan *pl, hipStream_t st) {
     return SWIT_OK;
 }
 
-static int adjoint_loop(swit_plan *pl, hipStream_t st) {
+static int adjoint_loop(swit_plan *pl, hipStream_t st, ShotRange r = ShotRange{0, -1}) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
     for (int it = d.nt, k = 0; it >= 2; k ^= 1) {
-        it -= launch_backward(pl, it, bs.R[k], bs.R[k ^ 1], bs.Bk[k], bs.Bk[k ^ 1], st);
+        it -= launch_backward(pl, it, bs.R[k], bs.R[k ^ 1], bs.Bk[k], bs.Bk[k ^ 1], st, r);
         LAUNCH_CHECK();
     }
     return SWIT_OK;
@@ -998,11 +1016,11 @@ static int adjoint_epilogue(swit_plan *pl, hipStream_t st) {
 }
 
 // Capture one time loop (which = 0 forward, 1 backward) into an executable graph.
-static int capture(swit_plan *pl, hipGraphExec_t *exec, int which, int save) {
+static int capture(swit_plan *pl, hipGraphExec_t *exec, int which, int save, ShotRange r = ShotRange{0, -1}) {
     BatchedState &bs = pl->bat;
     hipGraph_t graph = nullptr;
     HIP_TRY(hipStreamBeginCapture(bs.cap, hipStreamCaptureModeThreadLocal));
-    int rc = which == 0 ? forward_loop(pl, save, bs.cap) : adjoint_loop(pl, bs.cap);
+    int rc = which == 0 ? forward_loop(pl, save, bs.cap, r) : adjoint_loop(pl, bs.cap, r);
     hipError_t e = hipStreamEndCapture(bs.cap, &graph);
     if (rc != SWIT_OK) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -1027,7 +1045,31 @@ static int run_loop(swit_plan *pl, int which, int save, hipStream_t st) {
         bs.pending.push_back(t);
         HIP_TRY(hipEventRecord(t.e0, st));
     }
-    if (bs.use_graph) {
+    if (bs.streams == 2 && pl->ns >= 2) {
+        // the first half of the shots on st, the second on the plan's second stream,
+        // forked from st and joined back into it
+        const int h = pl->ns / 2;
+        const ShotRange half[2] = {{0, h}, {h, pl->ns - h}};
+        if (!bs.side) {
+            HIP_TRY(hipStreamCreateWithFlags(&bs.side, hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&bs.fork, hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&bs.join, hipEventDisableTiming));
+        }
+        HIP_TRY(hipEventRecord(bs.fork, st));
+        HIP_TRY(hipStreamWaitEvent(bs.side, bs.fork, 0));
+        for (int k = 0; k < 2; ++k) {
+            hipStream_t sk = k == 0 ? st : bs.side;
+            if (bs.use_graph) {
+                hipGraphExec_t &g = which == 0 ? bs.gfwd_h[k][save ? 1 : 0] : bs.gbwd_h[k];
+                if (!g) TRY(capture(pl, &g, which, save, half[k]));
+                HIP_TRY(hipGraphLaunch(g, sk));
+            } else {
+                TRY(which == 0 ? forward_loop(pl, save, sk, half[k]) : adjoint_loop(pl, sk, half[k]));
+            }
+        }
+        HIP_TRY(hipEventRecord(bs.join, bs.side));
+        HIP_TRY(hipStreamWaitEvent(st, bs.join, 0));
+    } else if (bs.use_graph) {
         hipGraphExec_t &g = which == 0 ? bs.gfwd[save ? 1 : 0] : bs.gbwd;
         if (!g) TRY(capture(pl, &g, which, save));
         HIP_TRY(hipGraphLaunch(g, st));
@@ -1065,10 +1107,19 @@ static int batched_adjoint(swit_plan *pl, const float *src, double *grad, double
 // ---- options and measurement -----------------------------------------------------------
 
 // Captured graphs hold the launch arguments: drop them when those change.
+static void drop_half_graphs(BatchedState &bs) {
+    for (auto &row : bs.gfwd_h)
+        for (auto &g : row)
+            if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    for (auto &g : bs.gbwd_h)
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+}
+
 static void drop_graphs(BatchedState &bs) {
     for (auto &g : bs.gfwd)
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
     if (bs.gbwd) { (void)hipGraphExecDestroy(bs.gbwd); bs.gbwd = nullptr; }
+    drop_half_graphs(bs);
     if (bs.gex_f) { (void)hipGraphExecDestroy(bs.gex_f); bs.gex_f = nullptr; }
     if (bs.gex_b) { (void)hipGraphExecDestroy(bs.gex_b); bs.gex_b = nullptr; }
     ++bs.gen;
@@ -1243,6 +1294,10 @@ extern "C" int swit_plan_set_option(swit_plan *pl, int option, int value) {
         if (value != 1 && value != 2) return set_err(SWIT_EINVAL, "backward steps per launch must be 1 or 2");
         bs.bwd_steps = value;
         drop_graphs(bs);
+        return SWIT_OK;
+    case SWIT_OPT_STREAMS:
+        if (value != 1 && value != 2) return set_err(SWIT_EINVAL, "streams must be 1 or 2");
+        bs.streams = value;
         return SWIT_OK;
     default:
         return set_err(SWIT_EINVAL, "unknown option %d", option);

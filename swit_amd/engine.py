@@ -118,11 +118,13 @@ class Plan:
 
     def set_option(self, option, value):
         """option: 'graph' | 'profile' | 'fwd_group' | 'fwd_colgroup' | 'bwd_colgroup' |
-        'bwd_steps' | 'bwd_tile_x' | 'exact_fused' (batched path)."""
+        'bwd_steps' | 'bwd_tile_x' | 'exact_fused' | 'streams' (batched path).  'streams' 2
+        runs the two halves of the shots on two streams (same results as 1)."""
         code = {"graph": _native.SWIT_OPT_GRAPH, "profile": _native.SWIT_OPT_PROFILE,
                 "fwd_group": _native.SWIT_OPT_FWD_GROUP, "fwd_colgroup": _native.SWIT_OPT_FWD_COLGROUP,
                 "bwd_colgroup": _native.SWIT_OPT_BWD_COLGROUP, "bwd_steps": _native.SWIT_OPT_BWD_STEPS,
-                "bwd_tile_x": _native.SWIT_OPT_BWD_TILE_X, "exact_fused": _native.SWIT_OPT_EXACT_FUSED}[option]
+                "bwd_tile_x": _native.SWIT_OPT_BWD_TILE_X, "exact_fused": _native.SWIT_OPT_EXACT_FUSED,
+                "streams": _native.SWIT_OPT_STREAMS}[option]
         _native.check(self._lib.swit_plan_set_option(self._h, code, int(value)))
         if option == "bwd_steps":
             self.bwd_steps = int(value)

@@ -42,6 +42,9 @@ STRIPS = "f32"
 # Time the batched step kernels' launch configurations once per synthetic-data plan,
 # at its first model, and keep the fastest (the results do not depend on the choice).
 AUTOTUNE = True
+# Batched plans with two or more shots run the two halves of the shots on two streams,
+# so that each time step's two launches overlap (bit-identical results; 1 turns it off).
+STREAMS = 2
 
 _JOBS = {}         # synthetic-data plan per homepath
 _OBS_JOBS = {}     # observed-data plan (no strips) per homepath
@@ -66,6 +69,8 @@ class _Job:
         self.plan = Plan(inputs["nx"], inputs["nz"], inputs["dx"], inputs["dt"], inputs["nt"],
                          inputs["npml"], inputs["src_xz"], inputs["rec_xz"],
                          free_surface=inputs["free_surface"], path=PATH, strips=strips or STRIPS)
+        if PATH == "batched":
+            self.plan.set_option("streams", STREAMS)
         self.fwd_src = None          # source traces of the last strip-saving forward
         self.model = None            # (vp, rho) the plan currently holds
         self.tuned = None            # the autotuner's choice, once made

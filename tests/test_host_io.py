@@ -203,6 +203,10 @@ class _RecordingPlan:
         self.id = len([e for e in self.log if e[0] == "new"])
         self.log.append(("new", self.id))
         self.torch = torch
+        self.options = {}
+
+    def set_option(self, option, value):
+        self.options[option] = value
 
     def set_model(self, vp, rho):
         self.log.append(("model", self.id, float(vp.double().sum())))
@@ -249,6 +253,9 @@ def test_dropin_plan_bookkeeping(tmp_path, monkeypatch):
                                              ("tune", 0), ("forward", 0, 1)]
     assert [e[0] for e in log if e[1] == 1] == ["new", "model", "forward"]
     assert solver._OBS_JOBS[home].plan.strips == "none"
+    # both batched plans run the halves of the shots on two streams
+    assert solver._OBS_JOBS[home].plan.options == {"streams": 2}
+    assert solver._JOBS[home].plan.options == {"streams": 2}
     solver.forward(simu_true, 'obs', savesnap=0)           # same model: no new plan, no upload
     assert [e[0] for e in log if e[1] == 1] == ["new", "model", "forward", "forward"]
     preprocess.process_workflow(simu, optim, 'obs')
