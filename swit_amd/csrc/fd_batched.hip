@@ -1136,12 +1136,14 @@ struct BatchedState {
     hipGraphExec_t gfwd[2] = {nullptr, nullptr};
     hipGraphExec_t gbwd = nullptr;
     // two streams (SWIT_OPT_STREAMS 2): each half of the shots runs its time loops on a
-    // stream of its own, from graphs of its own ([half][save] and [half])
+    // stream of its own, from graphs of its own ([half][save] and [half]); the exact
+    // loops likewise ([forward/adjoint][half])
     int streams = 1;
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
     hipGraphExec_t gfwd_h[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     hipGraphExec_t gbwd_h[2] = {nullptr, nullptr};
+    hipGraphExec_t gex_h[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     int use_graph = 1;
     int gen = 0;                 // bumped when captured launch arguments change
     int profile = 0;             // bracket each time loop with a pair of events

@@ -159,6 +159,7 @@ constexpr int kExNL = bk::kZL * kExCX;
 // - on the tile: lu -= Dx^T r, lw -= Dz^T r; write lp, lu, lw and gK.
 // The tiling keeps the mirror rows away from window edges where a value they need
 // would lie outside the window: no tile starts at rows fs-1 .. fs+3.
+// One launch covers shots s0 .. s0 + gridDim.x / ntiles - 1.
 // LIn/LOut: [ns][3][nxp][ld] (lp, lu, lw), ping-pong; gk: [ns][nxp][ld];
 // g: [ns][ngmax][nt]; inj_*: receiver groups at recording cells of the tiles grown by 3.
 __device__ __forceinline__ float lane_up(float v) {
@@ -168,7 +169,7 @@ __device__ __forceinline__ float lane_dn(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
 }
 __global__ __launch_bounds__(kThreads) void k_ex_step(
-    Dims d, int it, TileMap tm, const float2 *__restrict__ KB, const float2 *__restrict__ CP,
+    Dims d, int it, TileMap tm, int s0, const float2 *__restrict__ KB, const float2 *__restrict__ CP,
     const float4 *__restrict__ CV, DSave D, const float *__restrict__ LIn, float *__restrict__ LOut,
     float *__restrict__ gk, const int *__restrict__ inj_gstart, const InjGroup *__restrict__ inj_groups,
     const RecRef *__restrict__ inj_list, int inj_cap, const float *__restrict__ g) {
@@ -178,6 +179,7 @@ __global__ __launch_bounds__(kThreads) void k_ex_step(
 
     int s, t;
     work_item(xcd_logical(blockIdx.x, gridDim.x), (int)(gridDim.x / tm.ntiles), tm, s, t);
+    s += s0;
     const TileBox tb = tile_box(d, tm, t);
     const int zo = tb.z0 - kHalo, xo = tb.x0 - kHalo;
     const int lane = threadIdx.x & 63;

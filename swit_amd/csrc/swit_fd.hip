@@ -798,6 +798,9 @@ static void batched_release(swit_plan *pl) {
             if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
     for (auto &g : bs.gbwd_h)
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    for (auto &row : bs.gex_h)
+        for (auto &g : row)
+            if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
     if (bs.gex_f) { (void)hipGraphExecDestroy(bs.gex_f); bs.gex_f = nullptr; }
     if (bs.gex_b) { (void)hipGraphExecDestroy(bs.gex_b); bs.gex_b = nullptr; }
     for (auto &t : bs.pending) { (void)hipEventDestroy(t.e0); (void)hipEventDestroy(t.e1); }
@@ -1015,6 +1018,26 @@ static int adjoint_epilogue(swit_plan *pl, hipStream_t st) {
     return SWIT_OK;
 }
 
+// Two streams: body(k, half, stream) runs the first half of the shots (k = 0) on st and
+// the second (k = 1) on the plan's second stream, forked from st and joined back into it.
+template <class Body>
+static int on_two_streams(swit_plan *pl, hipStream_t st, Body body) {
+    BatchedState &bs = pl->bat;
+    const int h = pl->ns / 2;
+    const ShotRange half[2] = {{0, h}, {h, pl->ns - h}};
+    if (!bs.side) {
+        HIP_TRY(hipStreamCreateWithFlags(&bs.side, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&bs.fork, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&bs.join, hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventRecord(bs.fork, st));
+    HIP_TRY(hipStreamWaitEvent(bs.side, bs.fork, 0));
+    for (int k = 0; k < 2; ++k) TRY(body(k, half[k], k == 0 ? st : bs.side));
+    HIP_TRY(hipEventRecord(bs.join, bs.side));
+    HIP_TRY(hipStreamWaitEvent(st, bs.join, 0));
+    return SWIT_OK;
+}
+
 // Capture one time loop (which = 0 forward, 1 backward) into an executable graph.
 static int capture(swit_plan *pl, hipGraphExec_t *exec, int which, int save, ShotRange r = ShotRange{0, -1}) {
     BatchedState &bs = pl->bat;
@@ -1046,29 +1069,14 @@ static int run_loop(swit_plan *pl, int which, int save, hipStream_t st) {
         HIP_TRY(hipEventRecord(t.e0, st));
     }
     if (bs.streams == 2 && pl->ns >= 2) {
-        // the first half of the shots on st, the second on the plan's second stream,
-        // forked from st and joined back into it
-        const int h = pl->ns / 2;
-        const ShotRange half[2] = {{0, h}, {h, pl->ns - h}};
-        if (!bs.side) {
-            HIP_TRY(hipStreamCreateWithFlags(&bs.side, hipStreamNonBlocking));
-            HIP_TRY(hipEventCreateWithFlags(&bs.fork, hipEventDisableTiming));
-            HIP_TRY(hipEventCreateWithFlags(&bs.join, hipEventDisableTiming));
-        }
-        HIP_TRY(hipEventRecord(bs.fork, st));
-        HIP_TRY(hipStreamWaitEvent(bs.side, bs.fork, 0));
-        for (int k = 0; k < 2; ++k) {
-            hipStream_t sk = k == 0 ? st : bs.side;
-            if (bs.use_graph) {
-                hipGraphExec_t &g = which == 0 ? bs.gfwd_h[k][save ? 1 : 0] : bs.gbwd_h[k];
-                if (!g) TRY(capture(pl, &g, which, save, half[k]));
-                HIP_TRY(hipGraphLaunch(g, sk));
-            } else {
-                TRY(which == 0 ? forward_loop(pl, save, sk, half[k]) : adjoint_loop(pl, sk, half[k]));
-            }
-        }
-        HIP_TRY(hipEventRecord(bs.join, bs.side));
-        HIP_TRY(hipStreamWaitEvent(st, bs.join, 0));
+        TRY(on_two_streams(pl, st, [&](int k, ShotRange half, hipStream_t sk) -> int {
+            if (!bs.use_graph)
+                return which == 0 ? forward_loop(pl, save, sk, half) : adjoint_loop(pl, sk, half);
+            hipGraphExec_t &g = which == 0 ? bs.gfwd_h[k][save ? 1 : 0] : bs.gbwd_h[k];
+            if (!g) TRY(capture(pl, &g, which, save, half));
+            HIP_TRY(hipGraphLaunch(g, sk));
+            return SWIT_OK;
+        }));
     } else if (bs.use_graph) {
         hipGraphExec_t &g = which == 0 ? bs.gfwd[save ? 1 : 0] : bs.gbwd;
         if (!g) TRY(capture(pl, &g, which, save));
@@ -1113,6 +1121,9 @@ static void drop_half_graphs(BatchedState &bs) {
             if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
     for (auto &g : bs.gbwd_h)
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    for (auto &row : bs.gex_h)
+        for (auto &g : row)
+            if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
 }
 
 static void drop_graphs(BatchedState &bs) {
@@ -1289,6 +1300,9 @@ extern "C" int swit_plan_set_option(swit_plan *pl, int option, int value) {
     case SWIT_OPT_EXACT_FUSED:
         bs.exact_fused = value != 0;
         if (bs.gex_b) { (void)hipGraphExecDestroy(bs.gex_b); bs.gex_b = nullptr; }
+        for (auto &row : bs.gex_h)
+            for (auto &g : row)
+                if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
         return SWIT_OK;
     case SWIT_OPT_BWD_STEPS:
         if (value != 1 && value != 2) return set_err(SWIT_EINVAL, "backward steps per launch must be 1 or 2");
@@ -1673,52 +1687,59 @@ extern "C" int swit_group_gradient(swit_group *gr, int kind, const float *const 
 
 // ---- exact adjoint ---------------------------------------------------------------------------
 
-// Forward steps it = a+1..b (state after step a in F[(a+1) % 2]), storing the divergence
-// history as dsave says.
-static int exact_forward_steps(swit_plan *pl, int a, int b, DSave ds, hipStream_t st) {
+// Forward steps it = a+1..b (state after step a in F[(a+1) % 2]) of the shots in r,
+// storing the divergence history as dsave says.
+static int exact_forward_steps(swit_plan *pl, int a, int b, DSave ds, ShotRange r, hipStream_t st) {
     BatchedState &bs = pl->bat;
     for (int it = a + 1; it <= b; ++it) {
         const float *in = bs.F[it % 2];
         float *out = bs.F[(it + 1) % 2];
-        launch_fwd(pl, it, in, out, 0, st, ds, bs.src_ex);
+        launch_fwd(pl, it, in, out, 0, st, ds, bs.src_ex, r);
         LAUNCH_CHECK();
     }
     return SWIT_OK;
 }
 
-static int exact_forward_loop(swit_plan *pl, hipStream_t st) {
+// The whole forward of the shots in r: with the full divergence history, or keeping a
+// checkpoint of the state every ck steps.  Every buffer is shot-major, so the shots' part
+// of a checkpoint is one contiguous span.
+static int exact_forward_loop(swit_plan *pl, hipStream_t st, ShotRange range = ShotRange{0, -1}) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
-    if (!bs.ck) return exact_forward_steps(pl, 1, d.nt, DSave{bs.dhist, d.nt, 0}, st);
+    const ShotRange r = shots_of(pl, range);
+    if (!bs.ck) return exact_forward_steps(pl, 1, d.nt, DSave{bs.dhist, d.nt, 0}, r, st);
     // checkpoints: the state after step 1 + j ck (after step 1 it is zero)
-    const size_t chunk = (size_t)pl->ns * 3 * d.nxp * d.ld;
-    TRY(fill(bs.ckpt, chunk, st));
+    const size_t per = (size_t)3 * d.nxp * d.ld, chunk = pl->ns * per;
+    float *ck0 = bs.ckpt + r.s0 * per;
+    TRY(fill(ck0, r.n * per, st));
     for (int j = 1; j < bs.nck; ++j) {
         const int a = 1 + (j - 1) * bs.ck, b = a + bs.ck;
-        TRY(exact_forward_steps(pl, a, b, DSave{nullptr, 0, 0}, st));
-        TRY(copyk(bs.F[(b + 1) % 2], bs.ckpt + (size_t)j * chunk, chunk, st));
+        TRY(exact_forward_steps(pl, a, b, DSave{nullptr, 0, 0}, r, st));
+        TRY(copyk(bs.F[(b + 1) % 2] + r.s0 * per, ck0 + (size_t)j * chunk, r.n * per, st));
     }
     const int a = 1 + (bs.nck - 1) * bs.ck;
-    return exact_forward_steps(pl, a, d.nt, DSave{nullptr, 0, 0}, st);
+    return exact_forward_steps(pl, a, d.nt, DSave{nullptr, 0, 0}, r, st);
 }
 
-// Reverse steps b..a+1 of the exact adjoint, reading dJ/dseis from bs.gseis.
-// Reverse steps b..a+1 of the exact adjoint.  The fused kernel reads lam[k] and writes
-// lam[k^1] (lam[0] = bs.lam, lam[1] = bs.lam2); k carries over between segments.
-static int exact_adjoint_steps(swit_plan *pl, int a, int b, DSave ds, int &k, hipStream_t st) {
+// Reverse steps b..a+1 of the exact adjoint of the shots in r, reading dJ/dseis from
+// bs.gseis.  The fused kernel reads lam[k] and writes lam[k^1] (lam[0] = bs.lam,
+// lam[1] = bs.lam2); k carries over between segments.  The separate passes run on all
+// shots (r must cover them).
+static int exact_adjoint_steps(swit_plan *pl, int a, int b, DSave ds, int &k, ShotRange r, hipStream_t st) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
     const int ns = pl->ns;
     if (bs.exact_fused) {
         float *lam[2] = {bs.lam, bs.lam2};
         for (int it = b; it >= a + 1; --it, k ^= 1) {
-            hipLaunchKernelGGL(ex::k_ex_step, dim3(ns * bs.extm.ntiles), dim3(kThreads), 0, st, d, it, bs.extm, bs.KB,
-                               bs.CP, bs.CV, ds, lam[k], lam[k ^ 1], bs.gk, bs.exinj_start, bs.exinj_groups,
-                               bs.exinj_list, bs.exinj_cap, bs.gseis);
+            hipLaunchKernelGGL(ex::k_ex_step, dim3(r.n * bs.extm.ntiles), dim3(kThreads), 0, st, d, it, bs.extm,
+                               r.s0, bs.KB, bs.CP, bs.CV, ds, lam[k], lam[k ^ 1], bs.gk, bs.exinj_start,
+                               bs.exinj_groups, bs.exinj_list, bs.exinj_cap, bs.gseis);
             LAUNCH_CHECK();
         }
         return SWIT_OK;
     }
+    if (r.s0 != 0 || r.n != ns) return set_err(SWIT_EINVAL, "the separate exact passes run on all shots");
     const dim3 blk(64, 4), grd(cdiv(d.nzp, 64), cdiv(d.nxp, 4), ns);
     const dim3 gcol(cdiv(d.nxp, 256), ns), ginj(cdiv(d.ngmax, 256), ns);
     for (int it = b; it >= a + 1; --it) {
@@ -1735,28 +1756,29 @@ static int exact_adjoint_steps(swit_plan *pl, int a, int b, DSave ds, int &k, hi
     return SWIT_OK;
 }
 
-// The whole reverse sweep: from the full history, or segment by segment from the
-// checkpoints (each segment's forward is recomputed first, bit for bit).
-static int exact_adjoint_loop(swit_plan *pl, hipStream_t st) {
+// The whole reverse sweep of the shots in r: from the full history, or segment by segment
+// from the checkpoints (each segment's forward is recomputed first, bit for bit).
+static int exact_adjoint_loop(swit_plan *pl, hipStream_t st, ShotRange range = ShotRange{0, -1}) {
     BatchedState &bs = pl->bat;
     const Dims &d = pl->d;
+    const ShotRange r = shots_of(pl, range);
     int k = 0;                      // the sweep starts from lam[0] (both buffers are zero)
-    if (!bs.ck) return exact_adjoint_steps(pl, 1, d.nt, DSave{bs.dhist, d.nt, 0}, k, st);
-    const size_t chunk = (size_t)pl->ns * 3 * d.nxp * d.ld;
+    if (!bs.ck) return exact_adjoint_steps(pl, 1, d.nt, DSave{bs.dhist, d.nt, 0}, k, r, st);
+    const size_t per = (size_t)3 * d.nxp * d.ld, chunk = pl->ns * per;
     for (int j = bs.nck - 1; j >= 0; --j) {
         const int a = 1 + j * bs.ck, b = std::min(a + bs.ck, d.nt);
-        TRY(copyk(bs.ckpt + (size_t)j * chunk, bs.F[(a + 1) % 2], chunk, st));
-        TRY(exact_forward_steps(pl, a, b, DSave{bs.dhist, bs.ck, a}, st));
-        TRY(exact_adjoint_steps(pl, a, b, DSave{bs.dhist, bs.ck, a}, k, st));
+        TRY(copyk(bs.ckpt + (size_t)j * chunk + r.s0 * per, bs.F[(a + 1) % 2] + r.s0 * per, r.n * per, st));
+        TRY(exact_forward_steps(pl, a, b, DSave{bs.dhist, bs.ck, a}, r, st));
+        TRY(exact_adjoint_steps(pl, a, b, DSave{bs.dhist, bs.ck, a}, k, r, st));
     }
     return SWIT_OK;
 }
 
-static int exact_capture(swit_plan *pl, int which, hipGraphExec_t *exec) {
+static int exact_capture(swit_plan *pl, int which, hipGraphExec_t *exec, ShotRange r = ShotRange{0, -1}) {
     BatchedState &bs = pl->bat;
     hipGraph_t graph = nullptr;
     HIP_TRY(hipStreamBeginCapture(bs.cap, hipStreamCaptureModeThreadLocal));
-    int rc = which == 0 ? exact_forward_loop(pl, bs.cap) : exact_adjoint_loop(pl, bs.cap);
+    int rc = which == 0 ? exact_forward_loop(pl, bs.cap, r) : exact_adjoint_loop(pl, bs.cap, r);
     hipError_t e = hipStreamEndCapture(bs.cap, &graph);
     if (rc != SWIT_OK) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -1766,6 +1788,28 @@ static int exact_capture(swit_plan *pl, int which, hipGraphExec_t *exec) {
     e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return set_err(SWIT_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    return SWIT_OK;
+}
+
+// Run one exact loop (which = 0 forward, 1 adjoint) on st.  With two streams each half of
+// the shots runs on a stream of its own, as in run_loop (the adjoint only with the fused
+// kernel: the separate passes cover all shots).
+static int exact_run(swit_plan *pl, int which, hipStream_t st) {
+    BatchedState &bs = pl->bat;
+    if (bs.streams == 2 && pl->ns >= 2 && (which == 0 || bs.exact_fused)) {
+        return on_two_streams(pl, st, [&](int k, ShotRange half, hipStream_t sk) -> int {
+            if (!bs.use_graph)
+                return which == 0 ? exact_forward_loop(pl, sk, half) : exact_adjoint_loop(pl, sk, half);
+            hipGraphExec_t &g = bs.gex_h[which][k];
+            if (!g) TRY(exact_capture(pl, which, &g, half));
+            HIP_TRY(hipGraphLaunch(g, sk));
+            return SWIT_OK;
+        });
+    }
+    if (!bs.use_graph) return which == 0 ? exact_forward_loop(pl, st) : exact_adjoint_loop(pl, st);
+    hipGraphExec_t &g = which == 0 ? bs.gex_f : bs.gex_b;
+    if (!g) TRY(exact_capture(pl, which, &g));
+    HIP_TRY(hipGraphLaunch(g, st));
     return SWIT_OK;
 }
 
@@ -1780,12 +1824,7 @@ extern "C" int swit_forward_exact(swit_plan *pl, const float *src, float *seis, 
     bs.exact_valid = false;
     HIP_TRY(hipMemcpyAsync(bs.src_ex, src, sizeof(float) * pl->ns * d.nt, hipMemcpyDeviceToDevice, stream));
     TRY(forward_prologue(pl, 0, stream));
-    if (bs.use_graph) {
-        if (!bs.gex_f) TRY(exact_capture(pl, 0, &bs.gex_f));
-        HIP_TRY(hipGraphLaunch(bs.gex_f, stream));
-    } else {
-        TRY(exact_forward_loop(pl, stream));
-    }
+    TRY(exact_run(pl, 0, stream));
     hipLaunchKernelGGL(k_transpose, dim3(cdiv(d.ngmax, 32), cdiv(d.nt, 32), pl->ns), dim3(32, 8), 0, stream, d.nt,
                        d.ngmax, pl->seis_t, seis);
     LAUNCH_CHECK();
@@ -1812,12 +1851,7 @@ extern "C" int swit_adjoint_exact(swit_plan *pl, const float *gseis, double *gra
     TRY(fill(bs.lam, 3 * np * ns, stream));
     TRY(fill(bs.lam2, 3 * np * ns, stream));
     TRY(fill(bs.gk, np * ns, stream));
-    if (bs.use_graph) {
-        if (!bs.gex_b) TRY(exact_capture(pl, 1, &bs.gex_b));
-        HIP_TRY(hipGraphLaunch(bs.gex_b, stream));
-    } else {
-        TRY(exact_adjoint_loop(pl, stream));
-    }
+    TRY(exact_run(pl, 1, stream));
     hipLaunchKernelGGL(ex::k_ex_shotsum, dim3((unsigned)std::min<size_t>(cdiv((int)np, 256), 4096)), dim3(256), 0,
                        stream, d, ns, bs.gk, bs.gsum);
     LAUNCH_CHECK();

@@ -86,3 +86,44 @@ def test_two_streams_after_option_changes_and_profile():
     _, nb = p.kernel_time("backward")
     assert nf == w.nt - 1 and nb == w.nt - 1
     p.close()
+
+
+def _run_exact(w, mode, streams, graph, fused, src, obs):
+    import torch
+    from swit_amd.engine import Plan
+    p = Plan(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, w.src_xz, w.rec_xz, path="batched", exact=mode)
+    p.set_model(torch.from_numpy(w.vp_init), torch.from_numpy(workloads.gardner(w.vp_init)))
+    p.set_option("streams", streams)
+    p.set_option("graph", graph)
+    p.set_option("exact_fused", fused)
+    out = []
+    for rep in range(2):
+        syn = p.forward_exact(torch.from_numpy(src))
+        gseis = (syn - torch.from_numpy(obs).to(syn.device)).contiguous()
+        out.append((syn.cpu().numpy(), p.adjoint_exact(gseis).cpu().numpy()))
+    p.close()
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [True, "checkpoint"])
+@pytest.mark.parametrize("ns,nt,graph,fused", [
+    (2, 160, 1, 1),
+    (5, 161, 1, 1),
+    (4, 150, 0, 1),
+    (3, 151, 1, 0),                          # separate adjoint passes: the adjoint on one stream
+])
+def test_two_streams_exact_match_one_stream(mode, ns, nt, graph, fused):
+    """The exact forward and adjoint (full divergence history, or checkpoints with
+    segment recomputation) on two streams equal the one-stream run bit for bit."""
+    from oracle import oracle as O
+    w = workloads.small(seed=ns + nt, nx=52, nz=36, npml=10, nt=nt, ns=ns)
+    src = integrated_sources(w)
+    m = O.Model(w.vp_true, workloads.gardner(w.vp_true), w.dx, w.npml, "f32")
+    obs = O.forward(m, w.nt, w.dt, w.src_xz, w.rec_xz, src, save_strips=False)["seis"].astype(np.float32)
+    one = _run_exact(w, mode, 1, graph, fused, src, obs)
+    two = _run_exact(w, mode, 2, graph, fused, src, obs)
+    for rep in range(2):
+        np.testing.assert_array_equal(two[rep][0], one[rep][0], err_msg=f"seismograms, pass {rep}")
+        np.testing.assert_array_equal(two[rep][1], one[rep][1], err_msg=f"gradient, pass {rep}")
+    assert np.abs(one[0][1]).max() > 0
