@@ -68,8 +68,6 @@ SWIT_OPT_BWD_STEPS = 32
 SWIT_OPT_EXACT_FUSED = 64
 SWIT_OPT_BWD_TILE_X = 128
 SWIT_OPT_STREAMS = 256
-SWIT_OPT_BWD_BALANCE = 512
-SWIT_OPT_FWD_BALANCE = 1024
 
 
 class SwitPrecond(ctypes.Structure):

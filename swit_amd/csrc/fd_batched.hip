@@ -25,7 +25,6 @@
 #include "fd_common.h"
 #include "fd_kernels.h"
 
-#include <array>
 #include <vector>
 
 namespace swit {
@@ -116,14 +115,6 @@ __device__ __forceinline__ void work_item(int w, int nshot, const TileMap &tm, i
     s = rem / tiles;
     t = gq * per + rem % tiles;
 }
-
-// Split of one launch's logical work items over the XCDs: XCD k takes items
-// xs[k] .. xs[k+1]-1, its j-th block the item xs[k] + j (blocks past the range exit at
-// once); nsh = shots in the launch.
-struct XcdSplit {
-    int xs[kNumXcd + 1];
-    int nsh;
-};
 
 // Strip slot of a tile: slot index and source cell.
 struct StripRef {
@@ -288,8 +279,7 @@ struct DSave {
 // strips[(s * s_rows + s_row) * S] (the plan's strips with s_rows = nt, s_row = it - 1, or
 // a ring of rows that k_strip_encode turns into 16-bit rows).
 __global__ __launch_bounds__(kThreads) void k_fwd_fused(
-    Dims d, int it, TileMap tm, int s0, int ns, XcdSplit xp, const float2 *__restrict__ CP,
-    const float4 *__restrict__ CV,
+    Dims d, int it, TileMap tm, int s0, int ns, int G, const float2 *__restrict__ CP, const float4 *__restrict__ CV,
     const ShotGeom *__restrict__ geo, const float *__restrict__ src, const float *__restrict__ In,
     float *__restrict__ Out, const int *__restrict__ rec_start, const RecRef *__restrict__ rec_list,
     float *__restrict__ seis_t, const int *__restrict__ strip_start, const StripRef *__restrict__ strip_list,
@@ -300,16 +290,11 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
     __shared__ float sW[kNR];
     __shared__ float sQ[kNR];     // new p
 
-    // shots s0 .. s0 + ns - 1 in xp.nsh groups of near-equal size (at most G, the shots
-    // per workgroup), so that every workgroup of the launch carries about the same work
-    const int ngr = xp.nsh;
+    // shots s0 .. s0 + ns - 1 in ceil(ns / G) groups of near-equal size (at most G), so
+    // that every workgroup of the launch carries about the same work
+    const int ngr = (ns + G - 1) / G;
     int grp, t;
-    {
-        const int k = blockIdx.x % kNumXcd;
-        const int w = xp.xs[k] + (int)(blockIdx.x / kNumXcd);
-        if (w >= xp.xs[k + 1]) return;        // past this XCD's range: the whole workgroup
-        work_item(w, ngr, tm, grp, t);
-    }
+    work_item(xcd_logical(blockIdx.x, gridDim.x), ngr, tm, grp, t);
     const TileBox tb = tile_box(d, tm, t);
     const int zo = tb.z0 - kHalo, xo = tb.x0 - kHalo;
     const int tid = threadIdx.x;
@@ -476,7 +461,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
 // backward illumination), updated in the reference's order.  TX: tile width (columns).
 template <int SM, int TX>
 __global__ __launch_bounds__(kThreads) void k_bwd_fused(
-    Dims d, int it, TileMap tm, int s0, XcdSplit xp, const float2 *__restrict__ KB, const float2 *__restrict__ CP,
+    Dims d, int it, TileMap tm, int s0, const float2 *__restrict__ KB, const float2 *__restrict__ CP,
     const float4 *__restrict__ CV,
     const ShotGeom *__restrict__ geo, const float *__restrict__ src, const void *__restrict__ strips,
     const uint32_t *__restrict__ smax,
@@ -494,13 +479,8 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
     __shared__ float bW[kNL];
 
     int s, t;
-    {
-        const int k = blockIdx.x % kNumXcd;
-        const int w = xp.xs[k] + (int)(blockIdx.x / kNumXcd);
-        if (w >= xp.xs[k + 1]) return;        // past this XCD's range: the whole workgroup
-        work_item(w, xp.nsh, tm, s, t);
-    }
-    s += s0;                      // the launch covers shots s0 .. s0 + xp.nsh - 1
+    work_item(xcd_logical(blockIdx.x, gridDim.x), (int)(gridDim.x / tm.ntiles), tm, s, t);
+    s += s0;                      // the launch covers shots s0 .. s0 + gridDim.x / ntiles - 1
     const TileBox tb = tile_box(d, tm, t);
     const int zo = tb.z0 - kHalo, xo = tb.x0 - kHalo;
     const int lane = threadIdx.x & 63;
@@ -1164,11 +1144,6 @@ struct BatchedState {
     hipGraphExec_t gfwd_h[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     hipGraphExec_t gbwd_h[2] = {nullptr, nullptr};
     hipGraphExec_t gex_h[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    // work split of the forward and one-step backward kernels over the XCDs (1: by
-    // estimated cost, SWIT_OPT_FWD_BALANCE / SWIT_OPT_BWD_BALANCE) and the splits computed
-    // so far, keyed by {kernel, shots, shot units, column group, tile width, balanced}
-    int fwd_balance = 1, bwd_balance = 1;
-    std::vector<std::pair<std::array<int, 6>, XcdSplit>> splits;
     int use_graph = 1;
     int gen = 0;                 // bumped when captured launch arguments change
     int profile = 0;             // bracket each time loop with a pair of events

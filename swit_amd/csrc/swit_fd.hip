@@ -878,90 +878,6 @@ struct ShotRange {
 };
 static ShotRange shots_of(const swit_plan *pl, ShotRange r) { return r.n < 0 ? ShotRange{0, pl->ns} : r; }
 
-// ---- work split over the XCDs --------------------------------------------------------
-// A launch's logical items (work_item order) go to the 8 XCDs in contiguous ranges:
-// equal item counts, or (balanced) equal estimated cost, the bytes an item moves.  Items
-// differ: edge tiles are narrow, damped tiles also read the damping factors, tiles
-// without interior cells skip the backward's reconstruction and imaging, and forward
-// shot groups hold ceil or floor of ns/G shots.
-
-static int span(int a0, int a1, int lo, int hi) { return std::max(0, std::min(a1, hi) - std::max(a0, lo) + 1); }
-
-// One-step backward item (one shot, tile t): adjoint window in, {K, B} of the velocity
-// cells, the damping factors of a damped tile, the adjoint tile out; the interior parts
-// of the reconstruction window, its tile out and the accumulators' read-modify-write.
-static double bwd_item_cost(const Dims &d, const TileMap &tm, int t) {
-    const TileBox b = tile_box(d, tm, t);
-    const int npml = d.npml, zo = b.z0 - kHalo, xo = b.x0 - kHalo;
-    const double win = (double)span(zo, zo + bk::kZL - 1, 1, d.nzp) * span(xo, b.x1 + kHalo, 1, d.nxp);
-    const double vel = (double)span(b.z0 - 2, b.z1 + 1, 1, d.nzp) * span(b.x0 - 2, b.x1 + 1, 1, d.nxp);
-    const double tile = (double)(b.z1 - b.z0 + 1) * (b.x1 - b.x0 + 1);
-    const double win_i = (double)span(zo, zo + bk::kZL - 1, npml + 1, npml + d.nz) *
-                         span(xo, b.x1 + kHalo, npml + 1, npml + d.nx);
-    const double tile_i = (double)span(b.z0, b.z1, npml + 1, npml + d.nz) * span(b.x0, b.x1, npml + 1, npml + d.nx);
-    const bool damped = !(b.z0 - 2 >= npml + 1 && b.z0 + bk::kTZ + 1 <= npml + d.nz && b.x0 - 2 >= npml + 1 &&
-                          b.x1 + 2 <= npml + d.nx);
-    return 12 * win + 8 * vel + (damped ? 24 * vel : 0.0) + 12 * tile + 12 * win_i + 36 * tile_i;
-}
-
-// Forward item (a group of `shots` shots, tile t): per shot the window in and the tile
-// out; once per group the coefficients of the new-p region and of the tile.
-static double fwd_item_cost(const Dims &d, const TileMap &tm, int t, int shots) {
-    const TileBox b = tile_box(d, tm, t);
-    const int zo = b.z0 - kHalo, xo = b.x0 - kHalo;
-    const double win = (double)span(zo, zo + fk::kRZ - 1, 1, d.nzp) * span(xo, xo + fk::kRX - 1, 1, d.nxp);
-    const double reg = (double)span(b.z0 - 1, b.z0 + fk::kQZ - 2, 1, d.nzp) *
-                       span(b.x0 - 1, b.x0 + fk::kQX - 2, 1, d.nxp);
-    const double tile = (double)(b.z1 - b.z0 + 1) * (b.x1 - b.x0 + 1);
-    return shots * (12 * win + 12 * tile) + 8 * reg + 16 * tile;
-}
-
-// Split of a launch with `ngr` shot units (shots, or forward shot groups of near-equal
-// size over `nsh` shots) for the kernel `which` (0 forward, 1 one-step backward), cached
-// per {kernel, shots, units, column group, tile width, balanced}.  *grid: blocks to
-// launch (XCD k's j-th block takes item xs[k] + j).
-static XcdSplit xcd_split(BatchedState &bs, const Dims &d, const TileMap &tm, int which, int nsh, int ngr,
-                          bool balanced, int *grid) {
-    const std::array<int, 6> key = {which, nsh, ngr, tm.cg, tm.tx, balanced ? 1 : 0};
-    XcdSplit xp{};
-    bool found = false;
-    for (const auto &e : bs.splits)
-        if (e.first == key) { xp = e.second; found = true; break; }
-    if (!found) {
-        const int n = ngr * tm.ntiles;
-        xp.nsh = ngr;
-        if (!balanced) {
-            const int q = n / kNumXcd, r = n % kNumXcd;
-            for (int k = 0; k <= kNumXcd; ++k) xp.xs[k] = k * q + std::min(k, r);
-        } else {
-            std::vector<double> cum(n + 1, 0.0);
-            for (int w = 0; w < n; ++w) {
-                // the unit and tile of logical item w, as work_item maps them
-                const int per = tm.cg * tm.nzt, gq = w / (per * ngr), rem = w - gq * per * ngr;
-                const int cols = std::min(tm.cg, tm.nxt - gq * tm.cg), tiles = cols * tm.nzt;
-                const int u = rem / tiles, t = gq * per + rem % tiles;
-                const double c = which == 0 ? fwd_item_cost(d, tm, t, (u + 1) * nsh / ngr - u * nsh / ngr)
-                                            : bwd_item_cost(d, tm, t);
-                cum[w + 1] = cum[w] + c;
-            }
-            xp.xs[0] = 0;
-            for (int k = 1; k < kNumXcd; ++k) {
-                const double target = cum[n] * k / kNumXcd;
-                int w = xp.xs[k - 1];
-                while (w < n && cum[w + 1] <= target) ++w;
-                if (w < n && target - cum[w] > cum[w + 1] - target) ++w;     // the nearer boundary
-                xp.xs[k] = w;
-            }
-            xp.xs[kNumXcd] = n;
-        }
-        bs.splits.push_back({key, xp});
-    }
-    int most = 0;
-    for (int k = 0; k < kNumXcd; ++k) most = std::max(most, xp.xs[k + 1] - xp.xs[k]);
-    *grid = kNumXcd * most;
-    return xp;
-}
-
 static void launch_fwd(swit_plan *pl, int it, const float *in, float *out, int save, hipStream_t st,
                        DSave dsave = DSave{nullptr, 0, 0}, const float *src = nullptr,
                        ShotRange range = ShotRange{0, -1}) {
@@ -971,11 +887,9 @@ static void launch_fwd(swit_plan *pl, int it, const float *in, float *out, int s
     float *strips = !save ? nullptr : bs.strip_mode == 0 ? pl->strips : bs.ring;
     const int s_rows = bs.strip_mode == 0 ? d.nt : kStripRing;
     const int s_row = bs.strip_mode == 0 ? it - 1 : (it - 1) % kStripRing;
-    int grid = 0;
-    const XcdSplit xp = xcd_split(bs, d, bs.fwd, 0, r.n, cdiv(r.n, bs.Gf), bs.fwd_balance != 0, &grid);
-    hipLaunchKernelGGL(k_fwd_fused, dim3(grid), dim3(kThreads), 0, st, d, it, bs.fwd, r.s0, r.n, xp, bs.CP, bs.CV,
-                       pl->geo, src ? src : bs.src, in, out, bs.rec_start, bs.rec_list, pl->seis_t,
-                       bs.strip_start, bs.strip_list, strips, s_rows, s_row, dsave);
+    hipLaunchKernelGGL(k_fwd_fused, dim3(cdiv(r.n, bs.Gf) * bs.fwd.ntiles), dim3(kThreads), 0, st, d, it, bs.fwd,
+                       r.s0, r.n, bs.Gf, bs.CP, bs.CV, pl->geo, src ? src : bs.src, in, out, bs.rec_start,
+                       bs.rec_list, pl->seis_t, bs.strip_start, bs.strip_list, strips, s_rows, s_row, dsave);
 }
 
 // 16-bit plans: encode strip rows first..last (at most kStripRing, all still in the ring).
@@ -998,11 +912,9 @@ static void launch_bwd(swit_plan *pl, int it, const float *ri, float *ro, const 
     const Dims &d = pl->d;
     const void *strips = SM == 0 ? static_cast<const void *>(pl->strips) : static_cast<const void *>(bs.strips16);
     auto kern = bs.bwd.tx == bk::kBTXs[0] ? k_bwd_fused<SM, bk::kBTXs[0]> : k_bwd_fused<SM, bk::kBTXs[1]>;
-    int grid = 0;
-    const XcdSplit xp = xcd_split(bs, d, bs.bwd, 1, r.n, r.n, bs.bwd_balance != 0, &grid);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), 0, st, d, it, bs.bwd, r.s0, xp, bs.KB, bs.CP, bs.CV,
-                       pl->geo, bs.src, strips, bs.smax, ri, ro, bi, bo, bs.inj_start, bs.inj_groups, bs.inj_list,
-                       pl->sadj_t, bs.acc);
+    hipLaunchKernelGGL(kern, dim3(r.n * bs.bwd.ntiles), dim3(kThreads), 0, st, d, it, bs.bwd, r.s0, bs.KB, bs.CP,
+                       bs.CV, pl->geo, bs.src, strips, bs.smax, ri, ro, bi, bo, bs.inj_start, bs.inj_groups,
+                       bs.inj_list, pl->sadj_t, bs.acc);
 }
 
 // Make width index wi (of bk::kBTXs) the one-step backward kernel's tiling; the column
@@ -1286,70 +1198,59 @@ extern "C" int swit_plan_autotune(swit_plan *pl, int steps, int *choice, double 
     constexpr int kRepeats = 3;      // each candidate keeps its best of three bursts
     double best_f = 1e30, best_b = 1e30;
     int bg = bs.Gf, bcf = bs.fwd.cg, bcb = bs.bwd.cg;
-    int bfbal = bs.fwd_balance;
     for (int G : kGroups) {
         if (G > pl->ns) continue;
         for (int cg : kCols) {
             if (cg > bs.fwd.nxt) continue;
-            for (int bal = 0; bal <= 1; ++bal) {
-                bs.Gf = G;
-                bs.ngroups_f = cdiv(pl->ns, G);
-                bs.fwd.cg = cg;
-                bs.fwd_balance = bal;
-                double t = 1e30;
-                for (int r = 0; r < kRepeats; ++r) {
-                    double tr;
-                    TRY(time_steps(pl, 0, steps, stream, &tr));
-                    t = std::min(t, tr);
-                }
-                if (t < best_f) { best_f = t; bg = G; bcf = cg; bfbal = bal; }
+            bs.Gf = G;
+            bs.ngroups_f = cdiv(pl->ns, G);
+            bs.fwd.cg = cg;
+            double t = 1e30;
+            for (int r = 0; r < kRepeats; ++r) {
+                double tr;
+                TRY(time_steps(pl, 0, steps, stream, &tr));
+                t = std::min(t, tr);
             }
+            if (t < best_f) { best_f = t; bg = G; bcf = cg; }
         }
     }
-    int bsteps = bs.bwd_steps, bwi = bs.bwd_wi, bbal = bs.bwd_balance;
+    int bsteps = bs.bwd_steps, bwi = bs.bwd_wi;
     const int wi0 = bs.bwd_wi;
     // the backward kernels read strips: a plan without them has no backward to tune
     const int sp_max = (pl->flags & SWIT_PLAN_NO_STRIPS) ? 0 : 2;
     for (int sp = 1; sp <= sp_max; ++sp) {
         if (sp == 2 && pl->d.nt < 3) continue;
         bs.bwd_steps = sp;
-        // the one-step kernel's tile width and XCD work split are candidates too; the
-        // two-step kernel keeps its own tiling and split
+        // the one-step kernel's tile width is a candidate too; the two-step kernel keeps
+        // its own tiling
         for (int wi = 0; wi < 2; ++wi) {
             if (sp == 2 && wi != wi0) continue;
             select_bwd_width(bs, wi);
-            for (int bal = 0; bal <= (sp == 1 ? 1 : 0); ++bal) {
-                if (sp == 1) bs.bwd_balance = bal;
-                for (int cg : kCols) {
-                    if (cg > bs.bwd.nxt || (sp == 2 && cg > bs.bwd2.nxt)) continue;
-                    bs.bwd.cg = bs.bwd2.cg = cg;
-                    double t = 1e30;
-                    for (int r = 0; r < kRepeats; ++r) {
-                        double tr;
-                        TRY(time_steps(pl, 1, steps, stream, &tr));
-                        t = std::min(t, tr);
-                    }
-                    if (t < best_b) { best_b = t; bcb = cg; bsteps = sp; bwi = wi; bbal = bs.bwd_balance; }
+            for (int cg : kCols) {
+                if (cg > bs.bwd.nxt || (sp == 2 && cg > bs.bwd2.nxt)) continue;
+                bs.bwd.cg = bs.bwd2.cg = cg;
+                double t = 1e30;
+                for (int r = 0; r < kRepeats; ++r) {
+                    double tr;
+                    TRY(time_steps(pl, 1, steps, stream, &tr));
+                    t = std::min(t, tr);
                 }
+                if (t < best_b) { best_b = t; bcb = cg; bsteps = sp; bwi = wi; }
             }
         }
     }
     bs.Gf = bg;
     bs.ngroups_f = cdiv(pl->ns, bg);
     bs.fwd.cg = bcf;
-    bs.fwd_balance = bfbal;
     select_bwd_width(bs, bwi);
     bs.bwd.cg = bs.bwd2.cg = bcb;
     bs.bwd_steps = bsteps;
-    bs.bwd_balance = bbal;
     drop_graphs(bs);
     choice[0] = bg;
     choice[1] = bcf;
     choice[2] = bcb;
     choice[3] = bsteps;
     choice[4] = bs.bwd.tx;
-    choice[5] = bbal;
-    choice[6] = bfbal;
     us[0] = best_f;
     us[1] = sp_max ? best_b : 0.0;
     return SWIT_OK;
@@ -1411,12 +1312,6 @@ extern "C" int swit_plan_set_option(swit_plan *pl, int option, int value) {
     case SWIT_OPT_STREAMS:
         if (value != 1 && value != 2) return set_err(SWIT_EINVAL, "streams must be 1 or 2");
         bs.streams = value;
-        return SWIT_OK;
-    case SWIT_OPT_BWD_BALANCE:
-    case SWIT_OPT_FWD_BALANCE:
-        if (value != 0 && value != 1) return set_err(SWIT_EINVAL, "balance must be 0 or 1");
-        (option == SWIT_OPT_BWD_BALANCE ? bs.bwd_balance : bs.fwd_balance) = value;
-        drop_graphs(bs);
         return SWIT_OK;
     default:
         return set_err(SWIT_EINVAL, "unknown option %d", option);
