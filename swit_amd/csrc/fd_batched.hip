@@ -517,6 +517,17 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
     // the windows), cells outside the grid or the interior zeroed; then coefficients,
     // strip inserts, accumulators ----
     {
+        // cells outside the grid or the interior are zeroed first: an LDS store issued
+        // after pending transfers would make the compiler wait for all of them
+#pragma unroll
+        for (int j = 0; j < kJS; ++j) {
+            const int c = wave + kWaves * j;
+            if (c >= kCX) continue;
+            const int x = xo + c, jx = x - npml;
+            const int l = LW(c, lane);
+            if (!(rz && jx >= 1 && jx <= nx)) rP[l] = rU[l] = rW[l] = 0.f;
+            if (!(zin && x >= 1 && x <= d.nxp)) bP[l] = bU[l] = bW[l] = 0.f;
+        }
         const float *rin = RIn + (size_t)s * 3 * ni;
         const float *bin = BIn + (size_t)s * 3 * np;
 #pragma unroll
@@ -524,22 +535,17 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
             const int c = wave + kWaves * j;
             if (c >= kCX) continue;
             const int x = xo + c, jx = x - npml;
-            const int l = LW(c, lane);
             if (rz && jx >= 1 && jx <= nx) {
                 const unsigned k = (unsigned)((jx - 1) * nz + (jz - 1));
                 __builtin_amdgcn_global_load_lds(rin + k, rP + LW(c, 0), 4, 0, 0);
                 __builtin_amdgcn_global_load_lds(rin + ni + k, rU + LW(c, 0), 4, 0, 0);
                 __builtin_amdgcn_global_load_lds(rin + 2 * ni + k, rW + LW(c, 0), 4, 0, 0);
-            } else {
-                rP[l] = rU[l] = rW[l] = 0.f;
             }
             if (zin && x >= 1 && x <= d.nxp) {
                 const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
                 __builtin_amdgcn_global_load_lds(bin + k, bP + LW(c, 0), 4, 0, 0);
                 __builtin_amdgcn_global_load_lds(bin + np + k, bU + LW(c, 0), 4, 0, 0);
                 __builtin_amdgcn_global_load_lds(bin + 2 * np + k, bW + LW(c, 0), 4, 0, 0);
-            } else {
-                bP[l] = bU[l] = bW[l] = 0.f;
             }
         }
     }
@@ -801,6 +807,17 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused2(
     // ---- entry loads: both windows straight into LDS, then coefficients, strip inserts
     // of both steps, accumulators ----
     {
+        // cells outside the grid or the interior are zeroed first: an LDS store issued
+        // after pending transfers would make the compiler wait for all of them
+#pragma unroll
+        for (int j = 0; j < kJS; ++j) {
+            const int c = wave + kWaves * j;
+            if (c >= kCX) continue;
+            const int x = xo + c, jx = x - npml;
+            const int l = LW(c, lane);
+            if (!(rz && jx >= 1 && jx <= nx)) rP[l] = rU[l] = rW[l] = 0.f;
+            if (!(zin && x >= 1 && x <= d.nxp)) bP[l] = bU[l] = bW[l] = 0.f;
+        }
         const float *rin = RIn + (size_t)s * 3 * ni;
         const float *bin = BIn + (size_t)s * 3 * np;
 #pragma unroll
@@ -808,22 +825,17 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused2(
             const int c = wave + kWaves * j;
             if (c >= kCX) continue;
             const int x = xo + c, jx = x - npml;
-            const int l = LW(c, lane);
             if (rz && jx >= 1 && jx <= nx) {
                 const unsigned k = (unsigned)((jx - 1) * nz + (jz - 1));
                 __builtin_amdgcn_global_load_lds(rin + k, rP + LW(c, 0), 4, 0, 0);
                 __builtin_amdgcn_global_load_lds(rin + ni + k, rU + LW(c, 0), 4, 0, 0);
                 __builtin_amdgcn_global_load_lds(rin + 2 * ni + k, rW + LW(c, 0), 4, 0, 0);
-            } else {
-                rP[l] = rU[l] = rW[l] = 0.f;
             }
             if (zin && x >= 1 && x <= d.nxp) {
                 const unsigned k = (unsigned)((x - 1) * d.ld + (z - 1));
                 __builtin_amdgcn_global_load_lds(bin + k, bP + LW(c, 0), 4, 0, 0);
                 __builtin_amdgcn_global_load_lds(bin + np + k, bU + LW(c, 0), 4, 0, 0);
                 __builtin_amdgcn_global_load_lds(bin + 2 * np + k, bW + LW(c, 0), 4, 0, 0);
-            } else {
-                bP[l] = bU[l] = bW[l] = 0.f;
             }
         }
     }
