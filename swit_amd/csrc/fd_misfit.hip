@@ -221,7 +221,54 @@ template <> __device__ __forceinline__ void dft_small<4, true>(const double2 *z,
 template <> __device__ __forceinline__ void dft_small<5, false>(const double2 *z, double2 *y) { dft_small5<false>(z, y); }
 template <> __device__ __forceinline__ void dft_small<5, true>(const double2 *z, double2 *y) { dft_small5<true>(z, y); }
 
-// One butterfly stage of radix R (2..5), in place: all reads, barrier, all writes, barrier.
+// W_R^m = cos(2 pi m / R) - i sin(2 pi m / R) for the composite butterflies (R = 10),
+// each constant the double nearest its exact value.
+template <int R> struct Wtab;
+template <> struct Wtab<10> {
+    static constexpr double a = 0.80901699437494742410, b = 0.30901699437494742410;   // cos 36, cos 72 deg
+    static constexpr double e = 0.58778525229247312917, f = 0.95105651629515357212;   // sin 36, sin 72 deg
+    static constexpr double c[10] = {1.0, a, b, -b, -a, -1.0, -a, -b, b, a};
+    static constexpr double s[10] = {0.0, e, f, f, e, 0.0, -e, -f, -f, -e};
+};
+// a W_R^m (conjugated for the inverse); multiples of a quarter turn are exact rotations
+template <int R, bool INV>
+__device__ __forceinline__ double2 twiddle_const(double2 a, int m) {
+    if (m == 0) return a;
+    if (4 * m == R) return rot<INV>(a);
+    if (2 * m == R) return make_double2(-a.x, -a.y);
+    if (4 * m == 3 * R) return rot<!INV>(a);
+    return cmul(a, make_double2(Wtab<R>::c[m], INV ? Wtab<R>::s[m] : -Wtab<R>::s[m]));
+}
+
+// R = A B point DFT from size-A DFTs over the B interleaved subsequences z[B n1 + n2],
+// the twiddles W_R^(n2 k1), then size-B DFTs: y[k1 + A k2] (Cooley-Tukey, in registers).
+template <int A, int B, bool INV>
+__device__ __forceinline__ void dft_comp(const double2 *z, double2 *y) {
+    double2 t[A * B];
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) {
+        double2 in[A], out[A];
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) in[n1] = z[B * n1 + n2];
+        dft_small<A, INV>(in, out);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) t[n2 * A + k1] = twiddle_const<A * B, INV>(out[k1], n2 * k1);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) {
+        double2 in[B], out[B];
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) in[n2] = t[n2 * A + k1];
+        dft_small<B, INV>(in, out);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) y[k1 + A * k2] = out[k2];
+    }
+}
+template <> __device__ __forceinline__ void dft_small<10, false>(const double2 *z, double2 *y) { dft_comp<5, 2, false>(z, y); }
+template <> __device__ __forceinline__ void dft_small<10, true>(const double2 *z, double2 *y) { dft_comp<5, 2, true>(z, y); }
+
+// One butterfly stage of radix R (2..5, 10), in place: all reads, barrier, all writes,
+// barrier.
 template <int NMAX, bool INV, int R>
 __device__ void fft_stage_bfly(FftLds<NMAX> &L, int n, int ns) {
     constexpr int T = fft_threads<NMAX>();
@@ -237,13 +284,15 @@ __device__ void fft_stage_bfly(FftLds<NMAX> &L, int n, int ns) {
             const int jm = b % ns;
             o0[k] = (b - jm) * R + jm;
             // twiddle powers w^q from w = W^(jm n/(ns R)) by products
-            double2 z[R], wq[R];
-            wq[1] = fft_w<NMAX, INV>(L, jm * mult);
-#pragma unroll
-            for (int q = 2; q < R; ++q) wq[q] = cmul(wq[q - 1], wq[1]);
+            double2 z[R];
+            const double2 w1 = fft_w<NMAX, INV>(L, jm * mult);
+            double2 wq = w1;
             z[0] = L.x[b];
 #pragma unroll
-            for (int q = 1; q < R; ++q) z[q] = cmul(L.x[b + q * nb], wq[q]);
+            for (int q = 1; q < R; ++q) {
+                if (q >= 2) wq = cmul(wq, w1);
+                z[q] = cmul(L.x[b + q * nb], wq);
+            }
             dft_small<R, INV>(z, y[k]);
         }
     }
@@ -308,6 +357,7 @@ __device__ void fft_lds(FftLds<NMAX> &L, const FftFactors &f) {
         case 3: fft_stage_bfly<NMAX, INV, 3>(L, n, ns); break;
         case 4: fft_stage_bfly<NMAX, INV, 4>(L, n, ns); break;
         case 5: fft_stage_bfly<NMAX, INV, 5>(L, n, ns); break;
+        case 10: fft_stage_bfly<NMAX, INV, 10>(L, n, ns); break;
         default: fft_stage_gather<NMAX, INV>(L, n, R, ns); break;
         }
         ns *= R;
