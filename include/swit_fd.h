@@ -145,18 +145,23 @@ const char *swit_last_error(void);
                                    from and joined back into the caller's stream, so
                                    that each step's two launches overlap. Results are
                                    bit-identical to one stream.                        */
+#define SWIT_OPT_BWD_BALANCE 512 /* split of the one-step backward kernel's work items
+                                   over the 8 XCDs: 1 (default) contiguous ranges of
+                                   equal estimated bytes moved; 0 equal item counts
+                                   (same results)                                      */
 int swit_plan_set_option(swit_plan *plan, int option, int value);
 
 /* Autotuner (no counterpart in the reference): times `steps` back-to-back launches of
  * each batched step kernel (best of three bursts) on this plan's grid for every candidate forward
  * shot group size (1..8, 10, 12) x column-group order (forward with strip saving) and
  * every backward kernel (two steps per launch, or one step per launch at each tile
- * width) x column-group order, and keeps the fastest per time step.  Call between jobs
- * with the model set; it uses the wavefield and strip buffers as scratch, so saved
- * strips (and an exact-adjoint history) are invalidated.
- * choice[5] = {forward shots per workgroup, forward column group, backward column
- * group, backward steps per launch, one-step backward tile width}; us[2] = the chosen
- * forward / backward step-kernel time per time step in microseconds. */
+ * width and XCD work split) x column-group order, and keeps the fastest per time step.
+ * Call between jobs with the model set; it uses the wavefield and strip buffers as
+ * scratch, so saved strips (and an exact-adjoint history) are invalidated.
+ * choice[6] = {forward shots per workgroup, forward column group, backward column
+ * group, backward steps per launch, one-step backward tile width, one-step backward
+ * XCD work split}; us[2] = the chosen forward / backward
+ * step-kernel time per time step in microseconds. */
 int swit_plan_autotune(swit_plan *plan, int steps, int *choice, double *us, hipStream_t stream);
 
 /* Accumulated duration and time-step count of the step kernels timed while

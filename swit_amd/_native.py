@@ -68,6 +68,7 @@ SWIT_OPT_BWD_STEPS = 32
 SWIT_OPT_EXACT_FUSED = 64
 SWIT_OPT_BWD_TILE_X = 128
 SWIT_OPT_STREAMS = 256
+SWIT_OPT_BWD_BALANCE = 512
 
 
 class SwitPrecond(ctypes.Structure):

@@ -25,6 +25,7 @@
 #include "fd_common.h"
 #include "fd_kernels.h"
 
+#include <array>
 #include <vector>
 
 namespace swit {
@@ -115,6 +116,14 @@ __device__ __forceinline__ void work_item(int w, int nshot, const TileMap &tm, i
     s = rem / tiles;
     t = gq * per + rem % tiles;
 }
+
+// Split of one launch's logical work items over the XCDs: XCD k takes items
+// xs[k] .. xs[k+1]-1, its j-th block the item xs[k] + j (blocks past the range exit at
+// once); nsh = shots in the launch.
+struct XcdSplit {
+    int xs[kNumXcd + 1];
+    int nsh;
+};
 
 // Strip slot of a tile: slot index and source cell.
 struct StripRef {
@@ -461,7 +470,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
 // backward illumination), updated in the reference's order.  TX: tile width (columns).
 template <int SM, int TX>
 __global__ __launch_bounds__(kThreads) void k_bwd_fused(
-    Dims d, int it, TileMap tm, int s0, const float2 *__restrict__ KB, const float2 *__restrict__ CP,
+    Dims d, int it, TileMap tm, int s0, XcdSplit xp, const float2 *__restrict__ KB, const float2 *__restrict__ CP,
     const float4 *__restrict__ CV,
     const ShotGeom *__restrict__ geo, const float *__restrict__ src, const void *__restrict__ strips,
     const uint32_t *__restrict__ smax,
@@ -479,8 +488,13 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
     __shared__ float bW[kNL];
 
     int s, t;
-    work_item(xcd_logical(blockIdx.x, gridDim.x), (int)(gridDim.x / tm.ntiles), tm, s, t);
-    s += s0;                      // the launch covers shots s0 .. s0 + gridDim.x / ntiles - 1
+    {
+        const int k = blockIdx.x % kNumXcd;
+        const int w = xp.xs[k] + (int)(blockIdx.x / kNumXcd);
+        if (w >= xp.xs[k + 1]) return;        // past this XCD's range: the whole workgroup
+        work_item(w, xp.nsh, tm, s, t);
+    }
+    s += s0;                      // the launch covers shots s0 .. s0 + xp.nsh - 1
     const TileBox tb = tile_box(d, tm, t);
     const int zo = tb.z0 - kHalo, xo = tb.x0 - kHalo;
     const int lane = threadIdx.x & 63;
@@ -1156,6 +1170,11 @@ struct BatchedState {
     hipGraphExec_t gfwd_h[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     hipGraphExec_t gbwd_h[2] = {nullptr, nullptr};
     hipGraphExec_t gex_h[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    // work split of the one-step backward kernel over the XCDs (1: by estimated cost,
+    // SWIT_OPT_BWD_BALANCE) and the splits computed so far, keyed by {shots, column group,
+    // tile width, balanced}
+    int bwd_balance = 1;
+    std::vector<std::pair<std::array<int, 4>, XcdSplit>> splits;
     int use_graph = 1;
     int gen = 0;                 // bumped when captured launch arguments change
     int profile = 0;             // bracket each time loop with a pair of events

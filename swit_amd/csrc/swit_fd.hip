@@ -878,6 +878,74 @@ struct ShotRange {
 };
 static ShotRange shots_of(const swit_plan *pl, ShotRange r) { return r.n < 0 ? ShotRange{0, pl->ns} : r; }
 
+// ---- work split over the XCDs --------------------------------------------------------
+// A one-step backward launch's logical items (work_item order) go to the 8 XCDs in
+// contiguous ranges: equal item counts, or (balanced) equal estimated cost, the bytes an
+// item moves.  Items differ: edge tiles are narrow, damped tiles also read the damping
+// factors, and tiles without interior cells skip the reconstruction and imaging.
+
+static int span(int a0, int a1, int lo, int hi) { return std::max(0, std::min(a1, hi) - std::max(a0, lo) + 1); }
+
+// One item (one shot, tile t): adjoint window in, {K, B} of the velocity cells, the
+// damping factors of a damped tile, the adjoint tile out; the interior parts of the
+// reconstruction window, its tile out and the accumulators' read-modify-write.
+static double bwd_item_cost(const Dims &d, const TileMap &tm, int t) {
+    const TileBox b = tile_box(d, tm, t);
+    const int npml = d.npml, zo = b.z0 - kHalo, xo = b.x0 - kHalo;
+    const double win = (double)span(zo, zo + bk::kZL - 1, 1, d.nzp) * span(xo, b.x1 + kHalo, 1, d.nxp);
+    const double vel = (double)span(b.z0 - 2, b.z1 + 1, 1, d.nzp) * span(b.x0 - 2, b.x1 + 1, 1, d.nxp);
+    const double tile = (double)(b.z1 - b.z0 + 1) * (b.x1 - b.x0 + 1);
+    const double win_i = (double)span(zo, zo + bk::kZL - 1, npml + 1, npml + d.nz) *
+                         span(xo, b.x1 + kHalo, npml + 1, npml + d.nx);
+    const double tile_i = (double)span(b.z0, b.z1, npml + 1, npml + d.nz) * span(b.x0, b.x1, npml + 1, npml + d.nx);
+    const bool damped = !(b.z0 - 2 >= npml + 1 && b.z0 + bk::kTZ + 1 <= npml + d.nz && b.x0 - 2 >= npml + 1 &&
+                          b.x1 + 2 <= npml + d.nx);
+    return 12 * win + 8 * vel + (damped ? 24 * vel : 0.0) + 12 * tile + 12 * win_i + 36 * tile_i;
+}
+
+// Split of a launch of `nsh` shots, cached per {shots, column group, tile width,
+// balanced}.  *grid: blocks to launch (XCD k's j-th block takes item xs[k] + j).
+static XcdSplit bwd_xcd_split(BatchedState &bs, const Dims &d, const TileMap &tm, int nsh, int *grid) {
+    const bool balanced = bs.bwd_balance != 0;
+    const std::array<int, 4> key = {nsh, tm.cg, tm.tx, balanced ? 1 : 0};
+    XcdSplit xp{};
+    bool found = false;
+    for (const auto &e : bs.splits)
+        if (e.first == key) { xp = e.second; found = true; break; }
+    if (!found) {
+        const int n = nsh * tm.ntiles;
+        xp.nsh = nsh;
+        if (!balanced) {
+            const int q = n / kNumXcd, r = n % kNumXcd;
+            for (int k = 0; k <= kNumXcd; ++k) xp.xs[k] = k * q + std::min(k, r);
+        } else {
+            std::vector<double> cost(tm.ntiles);
+            for (int t = 0; t < tm.ntiles; ++t) cost[t] = bwd_item_cost(d, tm, t);
+            std::vector<double> cum(n + 1, 0.0);
+            for (int w = 0; w < n; ++w) {
+                // the tile of logical item w, as work_item maps it
+                const int per = tm.cg * tm.nzt, gq = w / (per * nsh), rem = w - gq * per * nsh;
+                const int cols = std::min(tm.cg, tm.nxt - gq * tm.cg), tiles = cols * tm.nzt;
+                cum[w + 1] = cum[w] + cost[gq * per + rem % tiles];
+            }
+            xp.xs[0] = 0;
+            for (int k = 1; k < kNumXcd; ++k) {
+                const double target = cum[n] * k / kNumXcd;
+                int w = xp.xs[k - 1];
+                while (w < n && cum[w + 1] <= target) ++w;
+                if (w < n && target - cum[w] > cum[w + 1] - target) ++w;     // the nearer boundary
+                xp.xs[k] = w;
+            }
+            xp.xs[kNumXcd] = n;
+        }
+        bs.splits.push_back({key, xp});
+    }
+    int most = 0;
+    for (int k = 0; k < kNumXcd; ++k) most = std::max(most, xp.xs[k + 1] - xp.xs[k]);
+    *grid = kNumXcd * most;
+    return xp;
+}
+
 static void launch_fwd(swit_plan *pl, int it, const float *in, float *out, int save, hipStream_t st,
                        DSave dsave = DSave{nullptr, 0, 0}, const float *src = nullptr,
                        ShotRange range = ShotRange{0, -1}) {
@@ -912,9 +980,11 @@ static void launch_bwd(swit_plan *pl, int it, const float *ri, float *ro, const 
     const Dims &d = pl->d;
     const void *strips = SM == 0 ? static_cast<const void *>(pl->strips) : static_cast<const void *>(bs.strips16);
     auto kern = bs.bwd.tx == bk::kBTXs[0] ? k_bwd_fused<SM, bk::kBTXs[0]> : k_bwd_fused<SM, bk::kBTXs[1]>;
-    hipLaunchKernelGGL(kern, dim3(r.n * bs.bwd.ntiles), dim3(kThreads), 0, st, d, it, bs.bwd, r.s0, bs.KB, bs.CP,
-                       bs.CV, pl->geo, bs.src, strips, bs.smax, ri, ro, bi, bo, bs.inj_start, bs.inj_groups,
-                       bs.inj_list, pl->sadj_t, bs.acc);
+    int grid = 0;
+    const XcdSplit xp = bwd_xcd_split(bs, d, bs.bwd, r.n, &grid);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), 0, st, d, it, bs.bwd, r.s0, xp, bs.KB, bs.CP, bs.CV,
+                       pl->geo, bs.src, strips, bs.smax, ri, ro, bi, bo, bs.inj_start, bs.inj_groups, bs.inj_list,
+                       pl->sadj_t, bs.acc);
 }
 
 // Make width index wi (of bk::kBTXs) the one-step backward kernel's tiling; the column
@@ -1214,28 +1284,31 @@ extern "C" int swit_plan_autotune(swit_plan *pl, int steps, int *choice, double 
             if (t < best_f) { best_f = t; bg = G; bcf = cg; }
         }
     }
-    int bsteps = bs.bwd_steps, bwi = bs.bwd_wi;
+    int bsteps = bs.bwd_steps, bwi = bs.bwd_wi, bbal = bs.bwd_balance;
     const int wi0 = bs.bwd_wi;
     // the backward kernels read strips: a plan without them has no backward to tune
     const int sp_max = (pl->flags & SWIT_PLAN_NO_STRIPS) ? 0 : 2;
     for (int sp = 1; sp <= sp_max; ++sp) {
         if (sp == 2 && pl->d.nt < 3) continue;
         bs.bwd_steps = sp;
-        // the one-step kernel's tile width is a candidate too; the two-step kernel keeps
-        // its own tiling
+        // the one-step kernel's tile width and XCD work split are candidates too; the
+        // two-step kernel keeps its own tiling and split
         for (int wi = 0; wi < 2; ++wi) {
             if (sp == 2 && wi != wi0) continue;
             select_bwd_width(bs, wi);
-            for (int cg : kCols) {
-                if (cg > bs.bwd.nxt || (sp == 2 && cg > bs.bwd2.nxt)) continue;
-                bs.bwd.cg = bs.bwd2.cg = cg;
-                double t = 1e30;
-                for (int r = 0; r < kRepeats; ++r) {
-                    double tr;
-                    TRY(time_steps(pl, 1, steps, stream, &tr));
-                    t = std::min(t, tr);
+            for (int bal = 0; bal <= (sp == 1 ? 1 : 0); ++bal) {
+                if (sp == 1) bs.bwd_balance = bal;
+                for (int cg : kCols) {
+                    if (cg > bs.bwd.nxt || (sp == 2 && cg > bs.bwd2.nxt)) continue;
+                    bs.bwd.cg = bs.bwd2.cg = cg;
+                    double t = 1e30;
+                    for (int r = 0; r < kRepeats; ++r) {
+                        double tr;
+                        TRY(time_steps(pl, 1, steps, stream, &tr));
+                        t = std::min(t, tr);
+                    }
+                    if (t < best_b) { best_b = t; bcb = cg; bsteps = sp; bwi = wi; bbal = bs.bwd_balance; }
                 }
-                if (t < best_b) { best_b = t; bcb = cg; bsteps = sp; bwi = wi; }
             }
         }
     }
@@ -1245,12 +1318,14 @@ extern "C" int swit_plan_autotune(swit_plan *pl, int steps, int *choice, double 
     select_bwd_width(bs, bwi);
     bs.bwd.cg = bs.bwd2.cg = bcb;
     bs.bwd_steps = bsteps;
+    bs.bwd_balance = bbal;
     drop_graphs(bs);
     choice[0] = bg;
     choice[1] = bcf;
     choice[2] = bcb;
     choice[3] = bsteps;
     choice[4] = bs.bwd.tx;
+    choice[5] = bbal;
     us[0] = best_f;
     us[1] = sp_max ? best_b : 0.0;
     return SWIT_OK;
@@ -1312,6 +1387,11 @@ extern "C" int swit_plan_set_option(swit_plan *pl, int option, int value) {
     case SWIT_OPT_STREAMS:
         if (value != 1 && value != 2) return set_err(SWIT_EINVAL, "streams must be 1 or 2");
         bs.streams = value;
+        return SWIT_OK;
+    case SWIT_OPT_BWD_BALANCE:
+        if (value != 0 && value != 1) return set_err(SWIT_EINVAL, "backward balance must be 0 or 1");
+        bs.bwd_balance = value;
+        drop_graphs(bs);
         return SWIT_OK;
     default:
         return set_err(SWIT_EINVAL, "unknown option %d", option);

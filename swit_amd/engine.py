@@ -118,13 +118,15 @@ class Plan:
 
     def set_option(self, option, value):
         """option: 'graph' | 'profile' | 'fwd_group' | 'fwd_colgroup' | 'bwd_colgroup' |
-        'bwd_steps' | 'bwd_tile_x' | 'exact_fused' | 'streams' (batched path).  'streams' 2
-        runs the two halves of the shots on two streams (same results as 1)."""
+        'bwd_steps' | 'bwd_tile_x' | 'bwd_balance' | 'exact_fused' | 'streams' (batched
+        path).  'streams' 2 runs the two halves of the shots on two streams; 'bwd_balance'
+        splits the one-step backward kernel's work over the XCDs by estimated cost (1, the
+        default) or by item count (0).  None of them changes results."""
         code = {"graph": _native.SWIT_OPT_GRAPH, "profile": _native.SWIT_OPT_PROFILE,
                 "fwd_group": _native.SWIT_OPT_FWD_GROUP, "fwd_colgroup": _native.SWIT_OPT_FWD_COLGROUP,
                 "bwd_colgroup": _native.SWIT_OPT_BWD_COLGROUP, "bwd_steps": _native.SWIT_OPT_BWD_STEPS,
                 "bwd_tile_x": _native.SWIT_OPT_BWD_TILE_X, "exact_fused": _native.SWIT_OPT_EXACT_FUSED,
-                "streams": _native.SWIT_OPT_STREAMS}[option]
+                "streams": _native.SWIT_OPT_STREAMS, "bwd_balance": _native.SWIT_OPT_BWD_BALANCE}[option]
         _native.check(self._lib.swit_plan_set_option(self._h, code, int(value)))
         if option == "bwd_steps":
             self.bwd_steps = int(value)
@@ -134,13 +136,14 @@ class Plan:
         and column-group order (backward: steps per launch, the one-step kernel's tile
         width and column-group order) and keep the fastest.  Invalidates saved strips.
         Returns {'fwd_group', 'fwd_colgroup', 'bwd_colgroup', 'bwd_steps', 'bwd_tile_x',
-        'fwd_us', 'bwd_us'} (times per time step)."""
-        choice = (ctypes.c_int * 5)()
+        'bwd_balance', 'fwd_us', 'bwd_us'} (times per time step)."""
+        choice = (ctypes.c_int * 6)()
         us = (ctypes.c_double * 2)()
         _native.check(self._lib.swit_plan_autotune(self._h, int(steps), choice, us, self._stream()))
         self.bwd_steps = choice[3]
         return {"fwd_group": choice[0], "fwd_colgroup": choice[1], "bwd_colgroup": choice[2],
-                "bwd_steps": choice[3], "bwd_tile_x": choice[4], "fwd_us": round(us[0], 2),
+                "bwd_steps": choice[3], "bwd_tile_x": choice[4], "bwd_balance": choice[5],
+                "fwd_us": round(us[0], 2),
                 "bwd_us": round(us[1], 2)}
 
     def kernel_time(self, which):

@@ -1,6 +1,6 @@
 """Two-step backward kernel (two reverse time steps per launch) against the one-step
-kernel at both its tile widths and the oracle: identical bits for odd and even step
-counts, every strip mode,
+kernel at both its tile widths and XCD work splits, and the oracle: identical bits for
+odd and even step counts, every strip mode,
 free-surface rows at many offsets inside the two-step windows, and receivers on every
 row of columns at and around tile edges (adjoint sources injected in both steps'
 regions, receivers sharing cells added in receiver order)."""
@@ -62,18 +62,21 @@ def test_two_step_backward_matches_one_step(nx, nz, npml, nt, ns, strips):
     m = O.Model(w.vp_true, workloads.gardner(w.vp_true), w.dx, w.npml, "f32")
     obs = O.forward(m, w.nt, w.dt, w.src_xz, w.rec_xz, src, save_strips=False)["seis"]
     out = {}
-    # the one-step kernel at both its tile widths, and the two-step kernel
-    for steps, tx in ((1, 16), (1, 24), (2, 24)):
+    # the one-step kernel at both its tile widths and every XCD work split, and the
+    # two-step kernel
+    cases = [(1, 16, 0), (1, 24, 0), (2, 24, 0), (1, 24, 1), (1, 16, 1)]
+    for steps, tx, bal in cases:
         for graph in (1, 0):
             p = _plan(w, strips, steps)
             p.set_option("bwd_tile_x", tx)
+            p.set_option("bwd_balance", bal)
             p.set_option("graph", graph)
             syn = p.forward(torch.from_numpy(src), save_strips=True)
             adj = (syn.cpu().numpy() - obs).astype(np.float32)
-            out[steps, tx, graph] = [t.cpu().numpy() for t in p.adjoint(torch.from_numpy(src),
-                                                                        torch.from_numpy(adj))]
+            out[steps, tx, bal, graph] = [t.cpu().numpy() for t in p.adjoint(torch.from_numpy(src),
+                                                                             torch.from_numpy(adj))]
             p.close()
-    ref = out[1, 24, 1]
+    ref = out[1, 24, 0, 1]
     for key, g in out.items():
         for x, y in zip(g, ref):
             np.testing.assert_array_equal(x, y, err_msg=str(key))
@@ -110,9 +113,11 @@ def test_two_step_is_the_default_and_profiles_per_step():
     assert p.bwd_steps == 2
     with pytest.raises(_native.SwitError):
         p.set_option("bwd_tile_x", 20)
+    with pytest.raises(_native.SwitError):
+        p.set_option("bwd_balance", 2)
     p.set_model(torch.from_numpy(w.vp_init), torch.from_numpy(workloads.gardner(w.vp_init)))
     tuned = p.autotune(steps=4)
-    assert tuned["bwd_tile_x"] in (16, 24) and tuned["bwd_steps"] in (1, 2)
+    assert tuned["bwd_tile_x"] in (16, 24) and tuned["bwd_steps"] in (1, 2) and tuned["bwd_balance"] in (0, 1)
     p.close()
 
 
@@ -135,14 +140,15 @@ def test_interior_tiles_with_buried_sources_and_receivers():
     m = O.Model(w.vp_true, workloads.gardner(w.vp_true), w.dx, w.npml, "f32")
     obs = O.forward(m, w.nt, w.dt, w.src_xz, w.rec_xz, src, save_strips=False)["seis"]
     out = {}
-    for steps, tx in ((1, 16), (1, 24), (2, 24)):
+    for steps, tx, bal in ((1, 16, 0), (1, 24, 0), (2, 24, 0), (1, 24, 1), (1, 16, 1)):
         p = _plan(w, "f32", steps)
         p.set_option("bwd_tile_x", tx)
+        p.set_option("bwd_balance", bal)
         syn = p.forward(torch.from_numpy(src), save_strips=True)
         adj = (syn.cpu().numpy() - obs).astype(np.float32)
-        out[steps, tx] = [t.cpu().numpy() for t in p.adjoint(torch.from_numpy(src), torch.from_numpy(adj))]
+        out[steps, tx, bal] = [t.cpu().numpy() for t in p.adjoint(torch.from_numpy(src), torch.from_numpy(adj))]
         p.close()
-    ref = out[1, 24]
+    ref = out[1, 24, 0]
     for key, g in out.items():
         for x, y in zip(g, ref):
             np.testing.assert_array_equal(x, y, err_msg=str(key))

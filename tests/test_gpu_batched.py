@@ -59,15 +59,19 @@ def test_group_sizes_and_replays_agree():
     w, src, adj, f, a = _case(seed=12, ns=5)
     p = _plan(w)
     ref = None
-    for G in (1, 3, 5, 2):
+    # shots per forward workgroup, with the two-step backward kernel or the one-step one
+    # and its XCD work split by estimated cost (1) or by item count (0)
+    for G, steps, bal in ((1, 2, 1), (3, 1, 1), (5, 1, 0), (2, 2, 0), (3, 1, 1)):
         p.set_option("fwd_group", G)
+        p.set_option("bwd_steps", steps)
+        p.set_option("bwd_balance", bal)
         for rep in range(2):
             seis, g = _run(p, src, adj)
             np.testing.assert_array_equal(seis, f["seis"])
             if ref is None:
                 ref = g
             for x, y in zip(g, ref):
-                assert rel_l2(x, y) == 0.0, (G, rep)
+                assert rel_l2(x, y) == 0.0, (G, steps, bal, rep)
     assert rel_l2(ref[0], a["grad"]) <= 1e-12
     p.close()
 
@@ -144,12 +148,13 @@ def test_autotune_keeps_results():
     ch = p.autotune(steps=5)
     print("autotune", ch)
     assert 1 <= ch["fwd_group"] <= w.ns and ch["fwd_colgroup"] >= 1 and ch["bwd_colgroup"] >= 1
+    assert ch["bwd_balance"] in (0, 1)
     assert ch["fwd_us"] > 0 and ch["bwd_us"] > 0
     with pytest.raises(SwitError):
         p.adjoint(torch.from_numpy(src), torch.from_numpy(adj))
     seis1, g1 = _run(p, src, adj)
     g_after = g1
-    for opt, v in (("fwd_colgroup", 1), ("bwd_colgroup", 3), ("fwd_group", 2)):
+    for opt, v in (("fwd_colgroup", 1), ("bwd_colgroup", 3), ("fwd_group", 2), ("bwd_balance", 0)):
         p.set_option(opt, v)
     seis2, g2 = _run(p, src, adj)
     p.close()

@@ -17,6 +17,7 @@ def _run(w, strips, steps, streams, graph, src, obs):
     p = Plan(w.nx, w.nz, w.dx, w.dt, w.nt, w.npml, w.src_xz, w.rec_xz, strips=strips)
     p.set_model(torch.from_numpy(w.vp_init), torch.from_numpy(workloads.gardner(w.vp_init)))
     p.set_option("bwd_steps", steps)
+    p.set_option("bwd_balance", 1 if steps == 1 else 0)
     p.set_option("streams", streams)
     p.set_option("graph", graph)
     out = []
