@@ -401,8 +401,8 @@ def main():
         return
 
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
-              "(shots shard across GPUs with one all-reduce per gradient; not built here)",
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(shots would shard across GPUs with one all-reduce per gradient; not implemented)",
               file=sys.stderr)
         sys.exit(2)
 
